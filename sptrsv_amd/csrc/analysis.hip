@@ -1,0 +1,532 @@
+// analysis.hip -- GPU level-set analysis (multirow_analysis_base_GPU,
+// analysis_csr.cu:272-410) producing bit-exact levels, iorder and the
+// (level, class) warp arrays, in both the reference's 32-lane layout and the
+// 64-lane layout the wave64 multirow solver uses.
+//
+// Pipeline (all on the device; the host reads back nlevs and the warp counts
+// only, to size arrays):
+//   1. levels      sync-free pass of the shared row engine with the max-plus
+//                  policy (analysis_csr.cu:104-185)
+//   2. nlevs       max reduction (thrust::max_element, :305)
+//   3. keys        key = 7*(lev-1) + class(off-diagonal count), bucket and
+//                  level histograms (:309-348)
+//   4. iorder      stable LSD radix sort of (key, row) pairs, 8-bit digits;
+//                  ties keep ascending row order (thrust::stable_sort_by_key,
+//                  :350-352)
+//   5. buckets     exclusive scan of the bucket histogram (:374)
+//   6. warps       ceil(count*size/lanes) per bucket, exclusive scan, per-warp
+//                  base row and size, sentinel ibase_row[n_warps] = n
+//                  (:354-385, kernel_base :243-270)
+#include <algorithm>
+#include <mutex>
+#include <unordered_set>
+#include <vector>
+
+#include "syncfree.hpp"
+
+namespace sptrsv {
+
+// Tile of the level pass from the integer average nnz/n, as
+// aux_call_kernel_analysis_L_coop_groups picks it (analysis_csr.cu:215-240):
+// 0-1 -> 1, 2 -> 2, 3-5 -> 4, 6-11 -> 8, 12-23 -> 16, else 32.
+static void solve_config_for_analysis(const sp_mat_t* m, int* lanes, int* wg) {
+    int avg = m->nr > 0 ? m->nnz / m->nr : 0;
+    *lanes = avg <= 1 ? 1 : avg == 2 ? 2 : avg <= 5 ? 4 : avg <= 11 ? 8 : avg <= 23 ? 16 : 32;
+    *wg = 256;
+}
+
+// ---- helpers -------------------------------------------------------------------
+
+// vect_size_calculator (analysis_csr.cu:12-31) on the off-diagonal count.
+__device__ __forceinline__ int row_class(int nz) {
+    if (nz == 0) return 6;
+    if (nz == 1) return 0;
+    if (nz <= 2) return 1;
+    if (nz <= 4) return 2;
+    if (nz <= 8) return 3;
+    if (nz <= 16) return 4;
+    return 5;
+}
+
+__global__ void k_fill_int(int* p, size_t count, int v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_max(const int* v, int n, int* out) {
+    __shared__ int s[256];
+    int m = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        m = max(m, v[i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, s[0]);
+}
+
+// key, bucket histogram [7*nlevs], level histogram [nlevs], row ids 0..n-1.
+// A level outside [1, nlevs] (impossible after a completed level pass) is
+// clamped and flagged in *bad instead of indexing out of the histograms.
+__global__ void k_keys(int n, const int* __restrict__ ia, const int* __restrict__ lev, int nlevs,
+                       int* __restrict__ key, int* __restrict__ vals, int* bucket_cnt,
+                       int* lev_size, int* bad) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int l = lev[i];
+        if (l < 1 || l > nlevs) {
+            atomicOr(bad, 1);
+            l = 1;
+        }
+        int kk = 7 * (l - 1) + row_class(ia[i + 1] - ia[i] - 1);
+        key[i] = kk;
+        vals[i] = i;
+        atomicAdd(&bucket_cnt[kk], 1);
+        atomicAdd(&lev_size[l - 1], 1);
+    }
+}
+
+// ---- exclusive scan (int), any length ------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ int block_excl_scan(int v, int* total) {
+    __shared__ int wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int i = 0; i < kScanThreads / 64; ++i) {
+        if (i < w) off += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+__global__ void k_scan_reduce(const int* in, long long n, int* sums) {
+    long long base = (long long)blockIdx.x * kScanTile;
+    int s = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        long long i = base + (long long)threadIdx.x * kScanItems + j;
+        if (i < n) s += in[i];
+    }
+    int tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_apply(const int* in, int* out, long long n, const int* offs) {
+    long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : 0;
+        s += v[j];
+    }
+    int tot;
+    int run = block_excl_scan(s, &tot) + (offs ? offs[blockIdx.x] : 0);
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+}
+
+// out[i] = sum_{j<i} in[j]; in and out may alias.
+static int exclusive_scan(const int* in, int* out, long long n, hipStream_t s) {
+    if (n <= 0) return SPTRSV_OK;
+    long long blocks = (n + kScanTile - 1) / kScanTile;
+    if (blocks == 1) {
+        hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kScanThreads), 0, s, in, out, n, nullptr);
+        SPT_HIP(hipGetLastError());
+        return SPTRSV_OK;
+    }
+    int* sums = nullptr;
+    SPT_HIP(hipMalloc(&sums, blocks * sizeof(int)));
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)blocks), dim3(kScanThreads), 0, s, in, n, sums);
+    int st = hipGetLastError() == hipSuccess ? exclusive_scan(sums, sums, blocks, s)
+                                             : fail(SPTRSV_E_HIP, "scan reduce launch");
+    if (st == SPTRSV_OK) {
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)blocks), dim3(kScanThreads), 0, s, in, out, n,
+                           sums);
+        if (hipGetLastError() != hipSuccess) st = fail(SPTRSV_E_HIP, "scan apply launch");
+    }
+    hipStreamSynchronize(s);
+    (void)hipFree(sums);
+    return st;
+}
+
+// ---- stable LSD radix sort of (key, value) pairs, 8-bit digits ----------------------
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+// hist[d * ntiles + tile] = number of digit-d keys in the tile.
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const int* keys, int n, int shift,
+                                                        int* hist, int ntiles) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kRsTile;
+    const int end = min(base + kRsTile, n);
+    for (int i = base + threadIdx.x; i < end; i += kRsThreads)
+        atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Scatter one tile in input order: rounds of 256 elements; within a round a
+// lane's rank among equal digits comes from 8 ballots (earlier lanes of its
+// wave) plus the counts of earlier waves; `run` carries earlier rounds.
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const int* __restrict__ kin,
+                                                           const int* __restrict__ vin,
+                                                           int* __restrict__ kout,
+                                                           int* __restrict__ vout, int n, int shift,
+                                                           const int* __restrict__ offs, int ntiles) {
+    __shared__ int run[256];
+    __shared__ int wcnt[kRsThreads / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    run[tid] = offs[tid * ntiles + blockIdx.x];
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int round = 0; round < kRsItems; ++round) {
+        for (int j = tid; j < (kRsThreads / 64) * 256; j += kRsThreads) (&wcnt[0][0])[j] = 0;
+        __syncthreads();
+        const int i = blockIdx.x * kRsTile + round * kRsThreads + tid;
+        const bool valid = i < n;
+        const int kv = valid ? kin[i] : 0;
+        const int d = (kv >> shift) & 255;
+        unsigned long long m = __ballot(valid);
+        for (int bit = 0; bit < 8; ++bit) {
+            unsigned long long bb = __ballot((d >> bit) & 1);
+            m &= ((d >> bit) & 1) ? bb : ~bb;
+        }
+        const int rank = __popcll(m & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(m);
+        __syncthreads();
+        if (valid) {
+            int off = run[d];
+            for (int q = 0; q < w; ++q) off += wcnt[q][d];
+            kout[off + rank] = kv;
+            vout[off + rank] = vin[i];
+        }
+        __syncthreads();
+        int add = 0;
+        for (int q = 0; q < kRsThreads / 64; ++q) add += wcnt[q][tid];
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+// Sorts (keys, vals) by key, stably; max_key bounds the keys.  The result is
+// left in (keys, vals); tk/tv are scratch of the same length.
+static int radix_sort_pairs(int* keys, int* vals, int* tk, int* tv, int n, int max_key,
+                            hipStream_t s) {
+    if (n <= 1) return SPTRSV_OK;
+    int bits = 0;
+    while (bits < 31 && (max_key >> bits) != 0) ++bits;
+    int passes = std::max(1, (bits + 7) / 8);
+    int ntiles = (n + kRsTile - 1) / kRsTile;
+    int* hist = nullptr;
+    SPT_HIP(hipMalloc(&hist, sizeof(int) * 256 * (size_t)ntiles));
+    int st = SPTRSV_OK;
+    int *ki = keys, *vi = vals, *ko = tk, *vo = tv;
+    for (int p = 0; p < passes && st == SPTRSV_OK; ++p) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kRsThreads), 0, s, ki, n, shift, hist, ntiles);
+        if (hipGetLastError() != hipSuccess) { st = fail(SPTRSV_E_HIP, "radix hist"); break; }
+        st = exclusive_scan(hist, hist, 256LL * ntiles, s);
+        if (st != SPTRSV_OK) break;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kRsThreads), 0, s, ki, vi, ko, vo, n, shift,
+                           hist, ntiles);
+        if (hipGetLastError() != hipSuccess) { st = fail(SPTRSV_E_HIP, "radix scatter"); break; }
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    if (st == SPTRSV_OK && ki != keys) {
+        if (hipMemcpyAsync(keys, ki, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(vals, vi, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            st = fail(SPTRSV_E_HIP, "radix copy back");
+    }
+    hipStreamSynchronize(s);
+    (void)hipFree(hist);
+    return st;
+}
+
+// ---- warp arrays ----------------------------------------------------------------------
+
+// get_size_from_pos + get_warps_per_bucks (analysis_csr.cu:66-101): size 2^class,
+// diagonal-only class counted as 1, warps = ceil(count*size/lanes).
+__global__ void k_warps_per_bucket(const int* cnt, int nb, int lanes, int* wpb) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b <= nb; b += gridDim.x * blockDim.x) {
+        if (b == nb) { wpb[b] = 0; continue; }
+        int cls = b % 7;
+        int size = (cls == 6) ? 1 : (1 << cls);
+        if (size > lanes) size = lanes;
+        long long t = (long long)cnt[b] * size;
+        wpb[b] = (int)((t + lanes - 1) / lanes);
+    }
+}
+
+// kernel_base (analysis_csr.cu:243-270): one wave per bucket.
+__global__ void k_warp_base(const int* warp_off, const int* bucket_off, int nb, int lanes,
+                            int* base, int* vsize, int n) {
+    const int lane = threadIdx.x & 63;
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nw = (gridDim.x * blockDim.x) >> 6;
+    if (wid == 0 && lane == 0) base[warp_off[nb]] = n;
+    for (int b = wid; b < nb; b += nw) {
+        int off = warp_off[b], cnt = warp_off[b + 1] - off;
+        int cls = b % 7;
+        int size = (cls == 6) ? 0 : (1 << cls);
+        if (size > lanes) size = lanes;
+        int sz = size ? size : 1;
+        int rows = bucket_off[b];
+        for (int x = lane; x < cnt; x += 64) {
+            vsize[off + x] = size;
+            base[off + x] = rows + x * (lanes / sz);
+        }
+    }
+}
+
+__global__ void k_inverse(const int* perm, int n, int* inv) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        inv[perm[i]] = i;
+}
+
+// warp_lev[l] = first warp of level l+1: warp_off[7*l], l = 0..nlevs.
+__global__ void k_warp_lev(const int* warp_off, int nlevs, int* warp_lev) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l <= nlevs; l += gridDim.x * blockDim.x)
+        warp_lev[l] = warp_off[7 * l];
+}
+
+static unsigned grid_for(long long count) {
+    long long g = (count + 255) / 256;
+    return (unsigned)std::max<long long>(1, std::min<long long>(g, 8192));
+}
+
+// ---- analysis-owned state -------------------------------------------------------------
+// info->priv points to an InfoPriv this library created; a caller struct that
+// was never filled here (malloc'ed, uninitialised) is recognised by the
+// pointer not being registered, so its garbage pointers are never freed.
+struct InfoPriv {
+    int unused = 0;
+};
+static std::mutex g_info_mu;
+static std::unordered_set<void*> g_infos;
+
+static InfoPriv* new_info_priv() {
+    InfoPriv* p = new InfoPriv();
+    std::lock_guard<std::mutex> g(g_info_mu);
+    g_infos.insert(p);
+    return p;
+}
+
+static bool owns_info(const dfr_analysis_info_t* info) {
+    std::lock_guard<std::mutex> g(g_info_mu);
+    return info->priv && g_infos.count(info->priv);
+}
+
+void free_info_arrays(dfr_analysis_info_t* info) {
+    if (!info || !owns_info(info)) return;
+    InfoPriv* ip = static_cast<InfoPriv*>(info->priv);
+    {
+        std::lock_guard<std::mutex> g(g_info_mu);
+        g_infos.erase(ip);
+    }
+    int* arrays[] = {info->lev_size, info->warp_lev, info->inv_iorder, info->iorder,
+                     info->ibase_row, info->ivect_size_warp, info->row_ctr, info->level,
+                     info->bucket_off, info->ibase_row64, info->ivect_size_wave};
+    for (int* p : arrays)
+        if (p) (void)hipFree(p);
+    delete ip;
+    MODE mode = info->mode;
+    *info = dfr_analysis_info_t{};
+    info->mode = mode;
+}
+
+// Warp arrays for one lane width.
+static int build_warps(const int* bucket_cnt, const int* bucket_off, int nb, int lanes, int n,
+                       int** base, int** vsize, int* count, int** warp_off_out, hipStream_t s) {
+    int* wpb = nullptr;
+    SPT_HIP(hipMalloc(&wpb, sizeof(int) * (size_t)(nb + 1)));
+    hipLaunchKernelGGL(k_warps_per_bucket, dim3(grid_for(nb + 1)), dim3(256), 0, s, bucket_cnt, nb,
+                       lanes, wpb);
+    SPT_HIP(hipGetLastError());
+    SPT_TRY(exclusive_scan(wpb, wpb, nb + 1, s));
+    int total = 0;
+    SPT_HIP(hipMemcpyAsync(&total, wpb + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPT_HIP(hipStreamSynchronize(s));
+    SPT_HIP(hipMalloc(base, sizeof(int) * (size_t)(total + 1)));
+    SPT_HIP(hipMalloc(vsize, sizeof(int) * (size_t)std::max(total, 1)));
+    hipLaunchKernelGGL(k_warp_base, dim3(grid_for((long long)nb * 64)), dim3(256), 0, s, wpb,
+                       bucket_off, nb, lanes, *base, *vsize, n);
+    SPT_HIP(hipGetLastError());
+    *count = total;
+    if (warp_off_out)
+        *warp_off_out = wpb;
+    else
+        SPT_HIP(hipFree(wpb));
+    return SPTRSV_OK;
+}
+
+static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
+    hipStream_t s = nullptr;
+    MatPriv* pv = priv_of(m);
+    const int n = m->nr;
+    free_info_arrays(info);
+    *info = dfr_analysis_info_t{};
+    info->priv = new_info_priv();
+    info->mode = mode;
+    info->n = n;
+    SPT_HIP(hipMalloc(&info->row_ctr, sizeof(int)));
+    SPT_HIP(hipMemset(info->row_ctr, 0, sizeof(int)));
+    if (n == 0) return SPTRSV_OK;
+
+    // 1. levels: sync-free max-plus pass (0 = unsolved).
+    SPT_HIP(hipMalloc(&info->level, sizeof(int) * (size_t)n));
+    int lanes = 2, wg = 256;
+    solve_config_for_analysis(m, &lanes, &wg);
+    {
+        SolveArgs A{};
+        A.n = n;
+        A.k = 1;
+        A.ia = m->ia;
+        A.ja = m->ja;
+        A.pub = info->level;
+        A.L = lanes;
+        int lg = 0;
+        while ((1 << (lg + 1)) <= lanes) ++lg;
+        A.lgL = lg;
+        A.ticket = pv->counters;
+        A.err = pv->counters + 1;
+        A.wd_ticks = pv->watchdog_ticks;
+        hipLaunchKernelGGL((k_prepare<uint32_t>), dim3(grid_for(n)), dim3(256), 0, s,
+                           reinterpret_cast<uint32_t*>(info->level), (size_t)n, 0u, pv->counters);
+        SPT_HIP(hipGetLastError());
+        int u = (pv->max_row > 2 * lanes) ? 4 : 1;
+        hipError_t e = launch_solve<PolLevel, 1>(KK_SIMPLE, 1, wg, u, A, s);
+        if (e != hipSuccess) return hip_fail(e, "level pass", __FILE__, __LINE__);
+    }
+    // 2. nlevs
+    int* d_max = nullptr;
+    SPT_HIP(hipMalloc(&d_max, sizeof(int) * 2));
+    SPT_HIP(hipMemsetAsync(d_max, 0, sizeof(int) * 2, s));
+    hipLaunchKernelGGL(k_max, dim3(std::min<unsigned>(grid_for(n), 1024)), dim3(256), 0, s,
+                       info->level, n, d_max);
+    SPT_HIP(hipGetLastError());
+    int h_max[2] = {0, 0};
+    SPT_HIP(hipMemcpyAsync(h_max, d_max, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    SPT_HIP(hipMemcpyAsync(&h_max[1], pv->counters + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPT_HIP(hipStreamSynchronize(s));
+    (void)hipFree(d_max);
+    if (h_max[1]) {
+        int zero = 0;
+        hipMemcpy(pv->counters + 1, &zero, sizeof(int), hipMemcpyHostToDevice);
+        return fail(SPTRSV_E_WATCHDOG, "level analysis: a dependency wait exceeded its bound");
+    }
+    const int nlevs = h_max[0];
+    if (nlevs <= 0 || (long long)nlevs * 7 > (1LL << 30))
+        return fail(SPTRSV_E_INVALID_CSR, "level analysis produced an invalid level count");
+    info->nlevs = nlevs;
+    info->lev_ctr = nlevs;
+    const int nb = 7 * nlevs;
+
+    // 3. keys and histograms
+    int *key = nullptr, *tk = nullptr, *tv = nullptr, *bucket_cnt = nullptr;
+    SPT_HIP(hipMalloc(&key, sizeof(int) * (size_t)n));
+    SPT_HIP(hipMalloc(&tk, sizeof(int) * (size_t)n));
+    SPT_HIP(hipMalloc(&tv, sizeof(int) * (size_t)n));
+    SPT_HIP(hipMalloc(&info->iorder, sizeof(int) * (size_t)n));
+    SPT_HIP(hipMalloc(&bucket_cnt, sizeof(int) * (size_t)(nb + 1)));
+    SPT_HIP(hipMalloc(&info->bucket_off, sizeof(int) * (size_t)(nb + 1)));
+    SPT_HIP(hipMalloc(&info->lev_size, sizeof(int) * (size_t)nlevs));
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(nb + 1)), dim3(256), 0, s, bucket_cnt, (size_t)(nb + 1), 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(nlevs)), dim3(256), 0, s, info->lev_size, (size_t)nlevs, 0);
+    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(256), 0, s, n, m->ia, info->level, nlevs, key,
+                       info->iorder, bucket_cnt, info->lev_size, pv->counters + 2);
+    SPT_HIP(hipGetLastError());
+    {
+        int bad = 0;
+        SPT_HIP(hipMemcpyAsync(&bad, pv->counters + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+        SPT_HIP(hipStreamSynchronize(s));
+        if (bad) {
+            (void)hipFree(key); (void)hipFree(tk); (void)hipFree(tv); (void)hipFree(bucket_cnt);
+            int zero = 0;
+            (void)hipMemcpy(pv->counters + 2, &zero, sizeof(int), hipMemcpyHostToDevice);
+            return fail(SPTRSV_E_HIP, "level pass left a row without a valid level");
+        }
+    }
+
+    // 4. iorder = stable sort of rows by key
+    int st = radix_sort_pairs(key, info->iorder, tk, tv, n, nb - 1, s);
+    (void)hipFree(tk);
+    (void)hipFree(tv);
+    (void)hipFree(key);
+    if (st != SPTRSV_OK) { (void)hipFree(bucket_cnt); return st; }
+
+    // 5. bucket offsets
+    SPT_TRY(exclusive_scan(bucket_cnt, info->bucket_off, nb + 1, s));
+    SPT_HIP(hipMalloc(&info->inv_iorder, sizeof(int) * (size_t)n));
+    hipLaunchKernelGGL(k_inverse, dim3(grid_for(n)), dim3(256), 0, s, info->iorder, n, info->inv_iorder);
+    SPT_HIP(hipGetLastError());
+
+    // 6. warp arrays (MULTIROW / ALL), 32-lane reference layout and 64-lane layout
+    if (mode == MULTIROW || mode == ALL) {
+        int* warp_off = nullptr;
+        st = build_warps(bucket_cnt, info->bucket_off, nb, 32, n, &info->ibase_row,
+                         &info->ivect_size_warp, &info->n_warps, &warp_off, s);
+        if (st == SPTRSV_OK) {
+            st = build_warps(bucket_cnt, info->bucket_off, nb, 64, n, &info->ibase_row64,
+                             &info->ivect_size_wave, &info->n_waves, nullptr, s);
+        }
+        if (st == SPTRSV_OK) {
+            if (hipMalloc(&info->warp_lev, sizeof(int) * (size_t)(nlevs + 1)) != hipSuccess) {
+                st = fail(SPTRSV_E_ALLOC, "warp_lev");
+            } else {
+                hipLaunchKernelGGL(k_warp_lev, dim3(grid_for(nlevs + 1)), dim3(256), 0, s, warp_off,
+                                   nlevs, info->warp_lev);
+            }
+        }
+        if (warp_off) (void)hipFree(warp_off);
+    }
+    hipFree(bucket_cnt);
+    if (st != SPTRSV_OK) return st;
+    SPT_HIP(hipStreamSynchronize(s));
+    return SPTRSV_OK;
+}
+
+}  // namespace sptrsv
+
+using namespace sptrsv;
+
+extern "C" {
+
+int multirow_analysis_base_GPU(dfr_analysis_info_t** info, sp_mat_t* gpu_L, MODE mode) {
+    if (!info || !gpu_L || !gpu_L->priv) return fail(SPTRSV_E_INVALID_ARG, "null info or matrix");
+    if (mode != ORDER && mode != MULTIROW && mode != ALL && mode != SIMPLE)
+        return fail(SPTRSV_E_INVALID_ARG, "unknown MODE");
+    if (!*info) {
+        *info = static_cast<dfr_analysis_info_t*>(calloc(1, sizeof(dfr_analysis_info_t)));
+        if (!*info) return fail(SPTRSV_E_ALLOC, "info struct");
+    }
+    int st = analyse(*info, gpu_L, mode);
+    if (st != SPTRSV_OK) free_info_arrays(*info);
+    return st;
+}
+
+int run_analysis(dfr_analysis_info_t* info, sp_mat_t* gpu_L) {
+    if (!info || !gpu_L) return fail(SPTRSV_E_INVALID_ARG, "null info or matrix");
+    if (info->mode == SIMPLE) return SPTRSV_E_NO_ANALYSIS_NEEDED;
+    return multirow_analysis_base_GPU(&info, gpu_L, info->mode);
+}
+
+}  // extern "C"

@@ -1,0 +1,516 @@
+// api_solve.hip -- C-ABI solve entry points (solver.h:5-7 and the mixed,
+// multi-RHS, autotune and bench additions), the two-pass refinement kernels
+// and the workspace management.
+#include <math.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "syncfree.hpp"
+
+namespace sptrsv {
+
+// ---- small device kernels -----------------------------------------------------
+
+__global__ void k_cvt_f64_f32(const double* in, float* out, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+// x = (double) x0
+__global__ void k_widen(const float* in, double* out, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+
+// FP64 update x += d
+__global__ void k_update(double* x, const float* d, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        x[i] += (double)d[i];
+}
+
+// FP64 residual r = b - L x over all entries of the row (diagonal included),
+// rounded to FP32 for the FP32 correction solve.  A group of L entry lanes x
+// K right-hand-side lanes per row, like the solves; no dependencies.
+template <int K>
+__global__ __launch_bounds__(256) void k_residual(int n, int k, const int* __restrict__ ia,
+                                                  const int* __restrict__ ja,
+                                                  const double* __restrict__ a,
+                                                  const double* __restrict__ b,
+                                                  const double* __restrict__ x,
+                                                  float* __restrict__ r, int L, int lgL) {
+    const int lgG = lgL + kLog2<K>();
+    const int G = 1 << lgG;
+    const int lane = threadIdx.x & 63;
+    const int gl = lane & (G - 1);
+    const int e = gl / K;
+    const int rr = gl % K;
+    const long long row = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> lgG;
+    const bool live = row < n && rr < k;
+    double s = 0.0;
+    if (live) {
+        const int rs = ia[row], re = ia[row + 1];
+        for (int p = rs + e; p < re; p += L) s = __fma_rn(a[p], x[(size_t)ja[p] * k + rr], s);
+    }
+    for (int o = K; o < G; o <<= 1) s += __shfl_xor(s, o);
+    if (live && e == 0) r[(size_t)row * k + rr] = (float)(b[(size_t)row * k + rr] - s);
+}
+
+__global__ void k_f64_to_f32_vals(const double* a, float* a32, int nnz) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+         i += (long long)gridDim.x * blockDim.x)
+        a32[i] = (float)a[i];
+}
+
+static unsigned stream_grid(size_t count, unsigned threads = 256) {
+    size_t g = (count + threads - 1) / threads;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 8192));
+}
+
+// ---- configuration --------------------------------------------------------------
+
+static int ilog2(int v) {
+    int l = 0;
+    while ((1 << (l + 1)) <= v) ++l;
+    return l;
+}
+static int pow2_ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Lanes per row from the integer average nnz/n, as solve_csr_simple.cu:121-137
+// picks its tile (0-2 -> 2, 3-5 -> 4, 6-11 -> 8, 12-23 -> 16, else 32).
+static int reference_lanes(const sp_mat_t* m) {
+    int avg = m->nr > 0 ? m->nnz / m->nr : 0;
+    if (avg <= 2) return 2;
+    if (avg <= 5) return 4;
+    if (avg <= 11) return 8;
+    if (avg <= 23) return 16;
+    return 32;
+}
+
+static bool has_order(const dfr_analysis_info_t* info) { return info && info->iorder; }
+static bool has_waves(const dfr_analysis_info_t* info) {
+    return info && info->ibase_row64 && info->ivect_size_wave && info->iorder;
+}
+
+// Entry window: 4 when rows are long compared with the lanes walking them.
+static int window_for(const MatPriv* pv, int lanes) { return (pv->max_row > 2 * lanes) ? 4 : 1; }
+
+int default_config(const sp_mat_t* m, const dfr_analysis_info_t* info, int precision, int k,
+                   sptrsv_config_t* out) {
+    (void)precision;
+    sptrsv_config_t c{};
+    c.kernel = SPTRSV_KERNEL_SIMPLE;
+    int K = pow2_ceil(k);
+    int lanes = reference_lanes(m);
+    while (lanes * K > kWave && lanes > 1) lanes >>= 1;
+    c.lanes = lanes;
+    c.wg = 256;
+    c.fused = 1;
+    if (info && info->mode == ORDER && has_order(info)) c.kernel = SPTRSV_KERNEL_ORDER;
+    if (info && info->mode == MULTIROW && has_waves(info)) c.kernel = SPTRSV_KERNEL_MULTIROW;
+    *out = c;
+    return SPTRSV_OK;
+}
+
+static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision, int k,
+                          const sptrsv_config_t* cfg, sptrsv_config_t* out) {
+    MatPriv* pv = priv_of(m);
+    if (cfg && cfg->kernel != SPTRSV_KERNEL_AUTO) {
+        *out = *cfg;
+    } else if (pv->tuned[precision] && pv->tuned_k[precision] == k &&
+               (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_SIMPLE ||
+                (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_ORDER && has_order(info)) ||
+                (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_MULTIROW && has_waves(info)))) {
+        *out = pv->tuned_cfg[precision];
+        if (cfg) out->fused = cfg->fused;
+    } else {
+        default_config(m, info, precision, k, out);
+        if (cfg) out->fused = cfg->fused;
+    }
+    int K = pow2_ceil(k);
+    if (out->lanes <= 0) {
+        sptrsv_config_t d;
+        default_config(m, info, precision, k, &d);
+        out->lanes = d.lanes;
+    }
+    if (out->wg != 256 && out->wg != 1024) out->wg = 256;
+    if (K > 1) out->wg = 256;
+    if (out->lanes > kWave / K) out->lanes = kWave / K;
+    if ((out->lanes & (out->lanes - 1)) != 0) return fail(SPTRSV_E_INVALID_ARG, "lanes must be a power of two");
+    if (out->kernel == SPTRSV_KERNEL_ORDER && !has_order(info))
+        return fail(SPTRSV_E_NOT_ANALYSED, "ORDER solve needs an analysis with iorder");
+    if (out->kernel == SPTRSV_KERNEL_MULTIROW && !has_waves(info))
+        return fail(SPTRSV_E_NOT_ANALYSED, "MULTIROW solve needs a MULTIROW/ALL analysis");
+    if (out->kernel < SPTRSV_KERNEL_SIMPLE || out->kernel > SPTRSV_KERNEL_MULTIROW)
+        return fail(SPTRSV_E_INVALID_ARG, "unknown kernel");
+    return SPTRSV_OK;
+}
+
+// ---- workspaces ---------------------------------------------------------------------
+
+static int reserve(sp_mat_t* m, int k, bool need_pub, bool need_two_pass) {
+    MatPriv* pv = priv_of(m);
+    size_t words = (size_t)m->nr * k;
+    if (need_pub && pv->pub_words < words) {
+        if (pv->pub) SPT_HIP(hipFree(pv->pub));
+        pv->pub = nullptr;
+        pv->pub_words = 0;
+        SPT_HIP(hipMalloc(&pv->pub, std::max<size_t>(words, 1) * sizeof(uint64_t)));
+        pv->pub_words = words;
+    }
+    if (need_two_pass && pv->resid_len < words) {
+        if (pv->b32) SPT_HIP(hipFree(pv->b32));
+        if (pv->corr) SPT_HIP(hipFree(pv->corr));
+        pv->b32 = nullptr;
+        pv->corr = nullptr;
+        pv->resid_len = 0;
+        SPT_HIP(hipMalloc(&pv->b32, std::max<size_t>(words, 1) * sizeof(float)));
+        SPT_HIP(hipMalloc(&pv->corr, std::max<size_t>(words, 1) * sizeof(float)));
+        pv->resid_len = words;
+    }
+    return SPTRSV_OK;
+}
+
+// ---- kernel timer ---------------------------------------------------------------------
+static std::mutex g_kt_mu;
+static bool g_kt_on = false;
+static std::vector<hipEvent_t> g_kt_pool;   // pairs
+static size_t g_kt_used = 0;
+
+// Next free (start, stop) event pair when timing is enabled, else nullptr.
+hipEvent_t* kernel_timer_pair(hipStream_t) {
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    if (!g_kt_on) return nullptr;
+    if (g_kt_used + 2 > g_kt_pool.size()) {
+        for (int i = 0; i < 64; ++i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            g_kt_pool.push_back(e);
+        }
+    }
+    hipEvent_t* p = &g_kt_pool[g_kt_used];
+    g_kt_used += 2;
+    return p;
+}
+
+// ---- one sync-free pass ----------------------------------------------------------------
+
+template <class P>
+static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_t& c, int k,
+                    const void* b, void* pub, double* x_out, hipStream_t s) {
+    MatPriv* pv = priv_of(m);
+    const int K = pow2_ceil(k);
+    SolveArgs A{};
+    A.n = m->nr;
+    A.k = k;
+    A.ia = m->ia;
+    A.ja = m->ja;
+    A.a64 = m->a;
+    A.a32 = m->a32;
+    A.b = b;
+    A.pub = pub;
+    A.x_out = x_out;
+    A.iorder = info ? info->iorder : nullptr;
+    A.wbase = info ? info->ibase_row64 : nullptr;
+    A.wsize = info ? info->ivect_size_wave : nullptr;
+    A.n_waves = info ? info->n_waves : 0;
+    A.L = c.lanes;
+    A.lgL = ilog2(c.lanes);
+    A.ticket = pv->counters;
+    A.err = pv->counters + 1;
+    A.wd_ticks = pv->watchdog_ticks;
+    size_t words = (size_t)m->nr * k;
+    using word = typename P::word;
+    hipLaunchKernelGGL((k_prepare<word>), dim3(stream_grid(words)), dim3(256), 0, s,
+                       static_cast<word*>(pub), words, P::kUnsolved, pv->counters);
+    SPT_HIP(hipGetLastError());
+    int kind = c.kernel == SPTRSV_KERNEL_ORDER ? KK_ORDER
+               : c.kernel == SPTRSV_KERNEL_MULTIROW ? KK_MULTIROW : KK_SIMPLE;
+    int u = window_for(pv, kind == KK_MULTIROW ? 32 : c.lanes);
+    hipEvent_t* ev = kernel_timer_pair(s);
+    if (ev) SPT_HIP(hipEventRecord(ev[0], s));
+    hipError_t e = launch_solve<P, 64>(kind, K, c.wg, u, A, s);
+    if (e != hipSuccess) return hip_fail(e, "sync-free solve launch", __FILE__, __LINE__);
+    if (ev) SPT_HIP(hipEventRecord(ev[1], s));
+    return SPTRSV_OK;
+}
+
+static int residual_to_f32(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b,
+                           const double* x, float* r, hipStream_t s) {
+    const int K = pow2_ceil(k);
+    int L = std::max(1, std::min(c.lanes, kWave / K));
+    int lgL = ilog2(L);
+    long long threads = ((long long)m->nr << (lgL + ilog2(K)));
+    unsigned grid = ceil_div(threads, 256);
+    if (grid == 0) return SPTRSV_OK;
+#define SPT_RES(KK)                                                                          \
+    hipLaunchKernelGGL((k_residual<KK>), dim3(grid), dim3(256), 0, s, m->nr, k, m->ia, m->ja, \
+                       m->a, b, x, r, L, lgL)
+    switch (K) {
+        case 1: SPT_RES(1); break;
+        case 2: SPT_RES(2); break;
+        case 4: SPT_RES(4); break;
+        case 8: SPT_RES(8); break;
+        case 16: SPT_RES(16); break;
+        case 32: SPT_RES(32); break;
+        default: SPT_RES(64); break;
+    }
+#undef SPT_RES
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+// The solve proper (no argument checks).
+static int solve_impl(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
+                      const sptrsv_config_t& c, const void* b, void* x, int k, int refine_steps,
+                      hipStream_t s) {
+    MatPriv* pv = priv_of(m);
+    size_t words = (size_t)m->nr * k;
+    if (precision == SPTRSV_PREC_F64)
+        return run_pass<PolF64>(m, info, c, k, b, x, nullptr, s);
+    if (precision == SPTRSV_PREC_F32)
+        return run_pass<PolF32>(m, info, c, k, b, x, nullptr, s);
+    // mixed precision
+    const double* bd = static_cast<const double*>(b);
+    double* xd = static_cast<double*>(x);
+    int extra = refine_steps;
+    if (c.fused) {
+        SPT_TRY(reserve(m, k, true, refine_steps > 1));
+        SPT_TRY(run_pass<PolMixed>(m, info, c, k, b, pv->pub, xd, s));
+        extra = refine_steps - 1;
+    } else {
+        SPT_TRY(reserve(m, k, false, true));
+        hipLaunchKernelGGL(k_cvt_f64_f32, dim3(stream_grid(words)), dim3(256), 0, s, bd, pv->b32, words);
+        SPT_HIP(hipGetLastError());
+        SPT_TRY(run_pass<PolF32>(m, info, c, k, pv->b32, pv->corr, nullptr, s));
+        hipLaunchKernelGGL(k_widen, dim3(stream_grid(words)), dim3(256), 0, s, pv->corr, xd, words);
+        SPT_HIP(hipGetLastError());
+    }
+    for (int it = 0; it < extra; ++it) {
+        SPT_TRY(residual_to_f32(m, c, k, bd, xd, pv->b32, s));
+        SPT_TRY(run_pass<PolF32>(m, info, c, k, pv->b32, pv->corr, nullptr, s));
+        hipLaunchKernelGGL(k_update, dim3(stream_grid(words)), dim3(256), 0, s, xd, pv->corr, words);
+        SPT_HIP(hipGetLastError());
+    }
+    return SPTRSV_OK;
+}
+
+static int check_solve_args(sp_mat_t* m, int precision, const void* b, void* x, int n, int k,
+                            int refine_steps) {
+    if (!m || !m->priv) return fail(SPTRSV_E_INVALID_ARG, "matrix not allocated (allocate_memory)");
+    if (n != m->nr) return fail(SPTRSV_E_INVALID_ARG, "n differs from the matrix row count");
+    if (k < 1 || k > kWave) return fail(SPTRSV_E_CAPACITY, "right-hand sides must be 1..64 per call");
+    if (n > 0 && (!b || !x)) return fail(SPTRSV_E_INVALID_ARG, "null b or x");
+    if (b == x && n > 0) return fail(SPTRSV_E_INVALID_ARG, "b and x must not alias");
+    if (precision == SPTRSV_PREC_F32) {
+        if (!m->a32) return fail(SPTRSV_E_INVALID_ARG, "matrix has no FP32 values");
+    } else if (precision == SPTRSV_PREC_F64 || precision == SPTRSV_PREC_MIXED) {
+        if (!m->a) return fail(SPTRSV_E_INVALID_ARG, "matrix has no FP64 values (allocated as FP32)");
+        if (precision == SPTRSV_PREC_MIXED && refine_steps < 1)
+            return fail(SPTRSV_E_INVALID_ARG, "refine_steps must be >= 1");
+    } else {
+        return fail(SPTRSV_E_INVALID_ARG, "unknown precision");
+    }
+    return SPTRSV_OK;
+}
+
+int solve_checked(sp_mat_t* m, dfr_analysis_info_t* info, int precision, const sptrsv_config_t* cfg,
+                  const void* b, void* x, int n, int k, int refine_steps, hipStream_t s) {
+    SPT_TRY(check_solve_args(m, precision, b, x, n, k, refine_steps));
+    if (n == 0) return SPTRSV_OK;
+    sptrsv_config_t c;
+    SPT_TRY(resolve_config(m, info, precision, k, cfg, &c));
+    return solve_impl(m, info, precision, c, b, x, k, refine_steps, s);
+}
+
+}  // namespace sptrsv
+
+using namespace sptrsv;
+
+extern "C" {
+
+int sptrsv_solve(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, const sptrsv_config_t* cfg,
+                 const void* b, void* x, int n, int k, int refine_steps, hipStream_t stream) {
+    return solve_checked(mat, info, precision, cfg, b, x, n, k, refine_steps, stream);
+}
+
+int sptrsv_default_config(const sp_mat_t* mat, const dfr_analysis_info_t* info, int precision, int k,
+                          sptrsv_config_t* out) {
+    if (!mat || !out || k < 1) return fail(SPTRSV_E_INVALID_ARG, "null argument");
+    return default_config(mat, info, precision, k, out);
+}
+
+int csr_L_solve_simple(sp_mat_t* mat, const double* b, double* x, int n, int* is_solved) {
+    (void)is_solved;
+    sptrsv_config_t c{SPTRSV_KERNEL_SIMPLE, 0, 0, 1};
+    return solve_checked(mat, nullptr, SPTRSV_PREC_F64, &c, b, x, n, 1, 1, nullptr);
+}
+
+int csr_L_solve_order(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b, double* x, int n,
+                      hipStream_t stream) {
+    sptrsv_config_t c{SPTRSV_KERNEL_ORDER, 0, 0, 1};
+    return solve_checked(mat, info, SPTRSV_PREC_F64, &c, b, x, n, 1, 1, stream);
+}
+
+int csr_L_solve_multirow(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b, double* x, int n,
+                         hipStream_t stream) {
+    sptrsv_config_t c{SPTRSV_KERNEL_MULTIROW, 0, 0, 1};
+    return solve_checked(mat, info, SPTRSV_PREC_F64, &c, b, x, n, 1, 1, stream);
+}
+
+int csr_L_solve_simple_f32(sp_mat_t* mat, const float* b, float* x, int n, int* is_solved) {
+    (void)is_solved;
+    sptrsv_config_t c{SPTRSV_KERNEL_SIMPLE, 0, 0, 1};
+    return solve_checked(mat, nullptr, SPTRSV_PREC_F32, &c, b, x, n, 1, 1, nullptr);
+}
+
+int csr_L_solve_order_f32(sp_mat_t* mat, dfr_analysis_info_t* info, const float* b, float* x, int n,
+                          hipStream_t stream) {
+    sptrsv_config_t c{SPTRSV_KERNEL_ORDER, 0, 0, 1};
+    return solve_checked(mat, info, SPTRSV_PREC_F32, &c, b, x, n, 1, 1, stream);
+}
+
+int csr_L_solve_multirow_f32(sp_mat_t* mat, dfr_analysis_info_t* info, const float* b, float* x,
+                             int n, hipStream_t stream) {
+    sptrsv_config_t c{SPTRSV_KERNEL_MULTIROW, 0, 0, 1};
+    return solve_checked(mat, info, SPTRSV_PREC_F32, &c, b, x, n, 1, 1, stream);
+}
+
+int csr_L_solve_mixed(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b, double* x, int n,
+                      int k, int refine_steps, hipStream_t stream) {
+    return solve_checked(mat, info, SPTRSV_PREC_MIXED, nullptr, b, x, n, k, refine_steps, stream);
+}
+
+int sptrsv_reserve(sp_mat_t* mat, int k) {
+    if (!mat || !mat->priv || k < 1 || k > kWave) return fail(SPTRSV_E_INVALID_ARG, "bad reserve");
+    return reserve(mat, k, true, true);
+}
+
+int sptrsv_check(sp_mat_t* mat, hipStream_t stream) {
+    if (!mat || !mat->priv) return fail(SPTRSV_E_INVALID_ARG, "matrix not allocated");
+    MatPriv* pv = priv_of(mat);
+    SPT_HIP(hipStreamSynchronize(stream));
+    int err = 0;
+    SPT_HIP(hipMemcpy(&err, pv->counters + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        int zero = 0;
+        SPT_HIP(hipMemcpy(pv->counters + 1, &zero, sizeof(int), hipMemcpyHostToDevice));
+        return fail(SPTRSV_E_WATCHDOG, "a dependency wait exceeded the watchdog bound (input "
+                                       "malformed or bound too small); affected rows hold NaN");
+    }
+    return SPTRSV_OK;
+}
+
+int sptrsv_kernel_timer(int enable) {
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    g_kt_on = enable != 0;
+    g_kt_used = 0;
+    return SPTRSV_OK;
+}
+
+int sptrsv_kernel_timer_read(float* total_ms, int* launches) {
+    if (!total_ms || !launches) return fail(SPTRSV_E_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    double tot = 0;
+    for (size_t i = 0; i + 1 < g_kt_used; i += 2) {
+        SPT_HIP(hipEventSynchronize(g_kt_pool[i + 1]));
+        float ms = 0;
+        SPT_HIP(hipEventElapsedTime(&ms, g_kt_pool[i], g_kt_pool[i + 1]));
+        tot += ms;
+    }
+    *total_ms = (float)tot;
+    *launches = (int)(g_kt_used / 2);
+    g_kt_used = 0;
+    return SPTRSV_OK;
+}
+
+int sptrsv_set_watchdog_ms(sp_mat_t* mat, double ms) {
+    if (!mat || !mat->priv || !(ms > 0)) return fail(SPTRSV_E_INVALID_ARG, "bad watchdog bound");
+    MatPriv* pv = priv_of(mat);
+    pv->watchdog_ticks = (long long)(ms * pv->wall_rate_khz);
+    if (pv->watchdog_ticks < 1) pv->watchdog_ticks = 1;
+    return SPTRSV_OK;
+}
+
+int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int k, int reps,
+                    sptrsv_config_t* best, float* best_ms) {
+    if (!mat || !mat->priv || reps < 1 || k < 1 || k > kWave || precision < 0 || precision > 2)
+        return fail(SPTRSV_E_INVALID_ARG, "bad autotune arguments");
+    MatPriv* pv = priv_of(mat);
+    const int n = mat->nr;
+    const int K = pow2_ceil(k);
+    size_t words = (size_t)n * k;
+    size_t esz = precision == SPTRSV_PREC_F32 ? sizeof(float) : sizeof(double);
+    void *b = nullptr, *x = nullptr;
+    SPT_HIP(hipMalloc(&b, std::max<size_t>(words, 1) * esz));
+    SPT_HIP(hipMalloc(&x, std::max<size_t>(words, 1) * esz));
+    SPT_HIP(hipMemset(b, 0, std::max<size_t>(words, 1) * esz));  // any finite right-hand side
+    if (precision == SPTRSV_PREC_MIXED) SPT_TRY(reserve(mat, k, true, true));
+
+    std::vector<sptrsv_config_t> cand;
+    int ref = std::min(reference_lanes(mat), kWave / K);
+    int lo = std::max(1, ref / 2), hi = std::min(kWave / K, ref * 4);
+    std::vector<int> wgs = (K == 1) ? std::vector<int>{256, 1024} : std::vector<int>{256};
+    for (int L = lo; L <= hi; L <<= 1)
+        for (int wg : wgs) cand.push_back({SPTRSV_KERNEL_SIMPLE, L, wg, 1});
+    if (has_order(info))
+        for (int L = lo; L <= hi; L <<= 1) cand.push_back({SPTRSV_KERNEL_ORDER, L, 256, 1});
+    if (has_waves(info))
+        for (int wg : wgs) cand.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1});
+
+    hipEvent_t e0, e1;
+    SPT_HIP(hipEventCreate(&e0));
+    SPT_HIP(hipEventCreate(&e1));
+    float best_t = 1e30f;
+    sptrsv_config_t bc = cand.front();
+    int status = SPTRSV_OK;
+    for (const auto& c : cand) {
+        // one warm-up, then reps timed solves
+        status = solve_impl(mat, info, precision, c, b, x, k, 1, nullptr);
+        if (status != SPTRSV_OK) break;
+        SPT_HIP(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps && status == SPTRSV_OK; ++r)
+            status = solve_impl(mat, info, precision, c, b, x, k, 1, nullptr);
+        SPT_HIP(hipEventRecord(e1, nullptr));
+        SPT_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= reps;
+        if (ms < best_t) {
+            best_t = ms;
+            bc = c;
+        }
+    }
+    hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(b);
+    (void)hipFree(x);
+    if (status != SPTRSV_OK) return status;
+    SPT_TRY(sptrsv_check(mat, nullptr));
+    pv->tuned[precision] = true;
+    pv->tuned_cfg[precision] = bc;
+    pv->tuned_k[precision] = k;
+    if (best) *best = bc;
+    if (best_ms) *best_ms = best_t;
+    return SPTRSV_OK;
+}
+
+}  // extern "C"
+
+namespace sptrsv {
+int convert_values_to_f32(sp_mat_t* m) {
+    if (m->nnz == 0) return SPTRSV_OK;
+    hipLaunchKernelGGL(k_f64_to_f32_vals, dim3(stream_grid((size_t)m->nnz)), dim3(256), 0, nullptr,
+                       m->a, m->a32, m->nnz);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+}  // namespace sptrsv

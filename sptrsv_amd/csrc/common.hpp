@@ -1,0 +1,64 @@
+// common.hpp -- shared host/device definitions of the gfx950 SpTRSV library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "sptrsv_amd.h"
+
+namespace sptrsv {
+
+// ---- error plumbing --------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define SPT_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) return ::sptrsv::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define SPT_TRY(call)                 \
+    do {                              \
+        int s_ = (call);              \
+        if (s_ != SPTRSV_OK) return s_; \
+    } while (0)
+
+// ---- constants -------------------------------------------------------------
+constexpr int kWave = 64;
+// Readiness is carried by the published word itself: the all-ones pattern (a
+// NaN payload that arithmetic never produces, see canon64/canon32) means
+// "row not solved yet".  Levels use 0 (levels are >= 1).
+constexpr uint64_t kUnsolved64 = ~0ull;
+constexpr uint32_t kUnsolved32 = ~0u;
+
+// ---- library-private state hung off sp_mat_t::priv ------------------------
+struct Workspace;  // defined in solve.hip
+
+struct MatPriv {
+    int* counters = nullptr;     // [0] ticket, [1] watchdog word, [2..] spare (device)
+    uint64_t* pub = nullptr;     // published words, n*k 64-bit (device)
+    size_t pub_words = 0;
+    double* resid = nullptr;     // two-pass refinement buffers (device)
+    float* corr = nullptr;
+    float* b32 = nullptr;
+    size_t resid_len = 0;
+    long long watchdog_ticks = 0;  // wall-clock ticks a wait may last
+    double wall_rate_khz = 100000.0;
+    int device = 0;
+    int max_row = 0;             // longest row (entries incl. diagonal)
+    bool tuned[3] = {false, false, false};
+    sptrsv_config_t tuned_cfg[3] = {};
+    int tuned_k[3] = {0, 0, 0};
+};
+
+MatPriv* priv_of(sp_mat_t* m);
+int ensure_priv(sp_mat_t* m);
+
+// ---- launch helpers ----------------------------------------------------------
+inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace sptrsv

@@ -1,0 +1,496 @@
+// interface.hip -- host interface (interface.h:6-9 / interface.cu:9-102),
+// error reporting, device helpers and the BENCH_RUN_SOLVE equivalent
+// (common.h:103-124).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sptrsv {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" +
+                    std::to_string(line) + ")";
+    g_last_error = m;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SPTRSV_E_ALLOC;
+    return SPTRSV_E_HIP;
+}
+
+MatPriv* priv_of(sp_mat_t* m) { return static_cast<MatPriv*>(m->priv); }
+
+int convert_values_to_f32(sp_mat_t* m);
+void free_info_arrays(dfr_analysis_info_t* info);
+
+static double watchdog_ms_default() {
+    const char* e = getenv("SPTRSV_WATCHDOG_MS");
+    if (e) {
+        double v = atof(e);
+        if (v > 0) return v;
+    }
+    return 20000.0;
+}
+
+int ensure_priv(sp_mat_t* m) {
+    if (m->priv) return SPTRSV_OK;
+    MatPriv* pv = new MatPriv();
+    SPT_HIP(hipGetDevice(&pv->device));
+    int rate = 0;
+    if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, pv->device) == hipSuccess &&
+        rate > 0)
+        pv->wall_rate_khz = rate;
+    pv->watchdog_ticks = (long long)(watchdog_ms_default() * pv->wall_rate_khz);
+    hipError_t e = hipMalloc(&pv->counters, 4 * sizeof(int));
+    if (e != hipSuccess) {
+        delete pv;
+        return hip_fail(e, "hipMalloc counters", __FILE__, __LINE__);
+    }
+    SPT_HIP(hipMemset(pv->counters, 0, 4 * sizeof(int)));
+    m->priv = pv;
+    return SPTRSV_OK;
+}
+
+static void free_mat(sp_mat_t* m) {
+    if (!m) return;
+    if (m->ia) (void)hipFree(m->ia);
+    if (m->ja) (void)hipFree(m->ja);
+    if (m->a) (void)hipFree(m->a);
+    if (m->a32) (void)hipFree(m->a32);
+    MatPriv* pv = priv_of(m);
+    if (pv) {
+        if (pv->counters) (void)hipFree(pv->counters);
+        if (pv->pub) (void)hipFree(pv->pub);
+        if (pv->b32) (void)hipFree(pv->b32);
+        if (pv->corr) (void)hipFree(pv->corr);
+        if (pv->resid) (void)hipFree(pv->resid);
+        delete pv;
+    }
+    memset(m, 0, sizeof(*m));
+}
+
+// Validates a host CSR lower triangle: every row non-empty, diagonal last,
+// off-diagonal columns in [0, i).  Returns the longest row or a status < 0.
+static int validate_host_csr(const int* ia, const int* ja, int n, int* max_row) {
+    if (n < 0) return fail(SPTRSV_E_INVALID_ARG, "n < 0");
+    if (n == 0) { *max_row = 0; return SPTRSV_OK; }
+    if (!ia || !ja) return fail(SPTRSV_E_INVALID_ARG, "null CSR arrays");
+    const int base = ia[0];
+    int mr = 0;
+    for (int i = 0; i < n; ++i) {
+        long long s = (long long)ia[i] - base, e = (long long)ia[i + 1] - base;
+        if (e <= s)
+            return fail(SPTRSV_E_INVALID_CSR, "row " + std::to_string(i) + " has no diagonal entry");
+        if (ja[e - 1] != i)
+            return fail(SPTRSV_E_INVALID_CSR,
+                        "row " + std::to_string(i) + ": last entry is not the diagonal");
+        for (long long p = s; p < e - 1; ++p)
+            if (ja[p] < 0 || ja[p] >= i)
+                return fail(SPTRSV_E_INVALID_CSR, "row " + std::to_string(i) + ": column " +
+                                                      std::to_string(ja[p]) +
+                                                      " outside the strict lower triangle");
+        mr = std::max<int>(mr, (int)(e - s));
+    }
+    *max_row = mr;
+    return SPTRSV_OK;
+}
+
+template <class V>
+static int upload(const int* ia, const int* ja, const V* val, int n, sp_mat_t* m) {
+    if (!m) return fail(SPTRSV_E_INVALID_ARG, "null sp_mat_t");
+    int max_row = 0;
+    memset(m, 0, sizeof(*m));
+    SPT_TRY(validate_host_csr(ia, ja, n, &max_row));
+    if (n > 0 && !val) return fail(SPTRSV_E_INVALID_ARG, "null values");
+    long long nnz = n > 0 ? (long long)ia[n] - ia[0] : 0;
+    if (nnz > 0x7fffffffLL) return fail(SPTRSV_E_CAPACITY, "nnz exceeds int32");
+    SPT_TRY(ensure_priv(m));
+    priv_of(m)->max_row = max_row;
+    m->nr = n;
+    m->nc = n;
+    m->nnz = (int)nnz;
+    std::vector<int> rebased;
+    const int* ia_up = ia;
+    if (n > 0 && ia[0] != 0) {
+        rebased.resize(n + 1);
+        for (int i = 0; i <= n; ++i) rebased[i] = ia[i] - ia[0];
+        ia_up = rebased.data();
+    }
+    const size_t off = n > 0 ? (size_t)ia[0] : 0;
+    int st = SPTRSV_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && st == SPTRSV_OK) st = hip_fail(e, what, __FILE__, __LINE__);
+    };
+    chk(hipMalloc(&m->ia, sizeof(int) * (size_t)(n + 1)), "hipMalloc ia");
+    chk(hipMalloc(&m->ja, sizeof(int) * (size_t)std::max<long long>(nnz, 1)), "hipMalloc ja");
+    chk(hipMalloc(&m->a32, sizeof(float) * (size_t)std::max<long long>(nnz, 1)), "hipMalloc a32");
+    if (st == SPTRSV_OK && n == 0) chk(hipMemset(m->ia, 0, sizeof(int)), "memset ia");
+    if (st == SPTRSV_OK && n > 0) {
+        chk(hipMemcpy(m->ia, ia_up, sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice), "copy ia");
+        chk(hipMemcpy(m->ja, ja + off, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice), "copy ja");
+    }
+    if (st == SPTRSV_OK) {
+        if constexpr (sizeof(V) == 8) {
+            chk(hipMalloc(&m->a, sizeof(double) * (size_t)std::max<long long>(nnz, 1)), "hipMalloc a");
+            if (st == SPTRSV_OK && nnz > 0)
+                chk(hipMemcpy(m->a, val + off, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice),
+                    "copy a");
+            if (st == SPTRSV_OK) st = convert_values_to_f32(m);
+        } else {
+            if (nnz > 0)
+                chk(hipMemcpy(m->a32, val + off, sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice),
+                    "copy a32");
+        }
+    }
+    if (st == SPTRSV_OK) chk(hipDeviceSynchronize(), "upload sync");
+    if (st != SPTRSV_OK) free_mat(m);
+    return st;
+}
+
+
+static __global__ void k_add_offset(int* p, size_t count, int off) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] += off;
+}
+
+static unsigned add_grid(size_t count) {
+    size_t g = (count + 255) / 256;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 8192));
+}
+
+// Uploads nmat host CSR lower triangles as one block-diagonal matrix with
+// global row and column indices; row_off receives the first row of each.
+int upload_concat(const int* n, const int* const* ia, const int* const* ja, const double* const* a,
+                  int nmat, sp_mat_t* m, std::vector<long long>& row_off) {
+    memset(m, 0, sizeof(*m));
+    row_off.assign(nmat + 1, 0);
+    std::vector<long long> nnz_off(nmat + 1, 0);
+    int max_row = 0;
+    for (int q = 0; q < nmat; ++q) {
+        int mr = 0;
+        if (n[q] < 0) return fail(SPTRSV_E_INVALID_ARG, "negative size");
+        int st = validate_host_csr(ia[q], ja[q], n[q], &mr);
+        if (st != SPTRSV_OK) return fail(st, "matrix " + std::to_string(q) + ": " + g_last_error);
+        max_row = std::max(max_row, mr);
+        row_off[q + 1] = row_off[q] + n[q];
+        nnz_off[q + 1] = nnz_off[q] + (n[q] > 0 ? (long long)ia[q][n[q]] - ia[q][0] : 0);
+    }
+    const long long rows = row_off[nmat], nnz = nnz_off[nmat];
+    if (rows > 0x7ffffffeLL || nnz > 0x7fffffffLL) return fail(SPTRSV_E_CAPACITY, "batch exceeds int32 indexing");
+    SPT_TRY(ensure_priv(m));
+    priv_of(m)->max_row = max_row;
+    m->nr = m->nc = (int)rows;
+    m->nnz = (int)nnz;
+    int st = SPTRSV_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && st == SPTRSV_OK) st = hip_fail(e, what, __FILE__, __LINE__);
+    };
+    chk(hipMalloc(&m->ia, sizeof(int) * (size_t)(rows + 1)), "hipMalloc ia");
+    chk(hipMalloc(&m->ja, sizeof(int) * (size_t)std::max<long long>(nnz, 1)), "hipMalloc ja");
+    chk(hipMalloc(&m->a, sizeof(double) * (size_t)std::max<long long>(nnz, 1)), "hipMalloc a");
+    chk(hipMalloc(&m->a32, sizeof(float) * (size_t)std::max<long long>(nnz, 1)), "hipMalloc a32");
+    for (int q = 0; q < nmat && st == SPTRSV_OK; ++q) {
+        if (n[q] == 0) continue;
+        const size_t nz = (size_t)(nnz_off[q + 1] - nnz_off[q]);
+        const size_t b0 = (size_t)ia[q][0];
+        chk(hipMemcpy(m->ia + row_off[q], ia[q], sizeof(int) * (size_t)n[q], hipMemcpyHostToDevice), "copy ia");
+        chk(hipMemcpy(m->ja + nnz_off[q], ja[q] + b0, sizeof(int) * nz, hipMemcpyHostToDevice), "copy ja");
+        chk(hipMemcpy(m->a + nnz_off[q], a[q] + b0, sizeof(double) * nz, hipMemcpyHostToDevice), "copy a");
+        if (st != SPTRSV_OK) break;
+        hipLaunchKernelGGL(k_add_offset, dim3(add_grid((size_t)n[q])), dim3(256), 0, nullptr,
+                           m->ia + row_off[q], (size_t)n[q], (int)(nnz_off[q] - ia[q][0]));
+        hipLaunchKernelGGL(k_add_offset, dim3(add_grid(nz)), dim3(256), 0, nullptr, m->ja + nnz_off[q], nz,
+                           (int)row_off[q]);
+        chk(hipGetLastError(), "offset kernels");
+    }
+    if (st == SPTRSV_OK) {
+        int last = (int)nnz;
+        chk(hipMemcpy(m->ia + rows, &last, sizeof(int), hipMemcpyHostToDevice), "copy ia end");
+    }
+    if (st == SPTRSV_OK) st = convert_values_to_f32(m);
+    if (st == SPTRSV_OK) chk(hipDeviceSynchronize(), "upload sync");
+    if (st != SPTRSV_OK) free_mat(m);
+    return st;
+}
+
+
+// ---- dependency latency probe ------------------------------------------------------
+// Block 0 and block `partner` (placed on another XCD by the round-robin
+// dispatch) hand a counter back and forth through agent-scope relaxed
+// stores and polling loads, the primitives of the sync-free solves.  Every
+// wait is bounded; out[0..1] = elapsed wall-clock ticks, out[2] = error.
+static __global__ void k_pingpong(unsigned* flags, int partner, int rounds, long long* out, int max_spin) {
+    const int b = blockIdx.x;
+    if ((b != 0 && b != partner) || threadIdx.x != 0) return;
+    unsigned* mine = flags + (b == 0 ? 0 : 32);
+    unsigned* other = flags + (b == 0 ? 32 : 0);
+    long long t0 = wall_clock64();
+    int err = 0;
+    for (int i = 0; i < rounds && !err; ++i) {
+        unsigned want = 2u * (unsigned)i + 1u;
+        if (b == 0) __hip_atomic_store(mine, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int s = 0;
+        while (__hip_atomic_load(other, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want)
+            if (++s > max_spin) { err = 1; break; }
+        if (b != 0 && !err) __hip_atomic_store(mine, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    long long t1 = wall_clock64();
+    if (b == 0) out[0] = t1 - t0;
+    if (err) out[2] = 1;
+}
+
+// hip events for sptrsv_timer_*
+static std::mutex g_timer_mu;
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+
+}  // namespace sptrsv
+
+using namespace sptrsv;
+
+extern "C" {
+
+const char* sptrsv_status_string(int status) {
+    switch (status) {
+        case SPTRSV_OK: return "ok";
+        case SPTRSV_E_NO_ANALYSIS_NEEDED: return "SIMPLE mode needs no analysis";
+        case SPTRSV_E_INVALID_ARG: return "invalid argument";
+        case SPTRSV_E_INVALID_CSR: return "invalid lower-triangular CSR";
+        case SPTRSV_E_ALLOC: return "device allocation failed";
+        case SPTRSV_E_HIP: return "HIP runtime error";
+        case SPTRSV_E_WATCHDOG: return "dependency wait exceeded its bound";
+        case SPTRSV_E_NOT_ANALYSED: return "solver needs an analysis";
+        case SPTRSV_E_UNAVAILABLE: return "optional runtime unavailable";
+        case SPTRSV_E_IO: return "Matrix Market read error";
+        case SPTRSV_E_CAPACITY: return "request exceeds a fixed capacity";
+        default: return "unknown status";
+    }
+}
+
+const char* sptrsv_last_error(void) { return g_last_error.c_str(); }
+
+int allocate_memory(const int* csrRowPtrL, const int* csrColIdxL, const double* csrValL, int n,
+                    sp_mat_t* gpu_L, MODE mode) {
+    (void)mode;
+    return upload(csrRowPtrL, csrColIdxL, csrValL, n, gpu_L);
+}
+
+int allocate_memory_f32(const int* csrRowPtrL, const int* csrColIdxL, const float* csrValL, int n,
+                        sp_mat_t* gpu_L, MODE mode) {
+    (void)mode;
+    return upload(csrRowPtrL, csrColIdxL, csrValL, n, gpu_L);
+}
+
+void free_device_analysis(sp_mat_t* gpu_L, dfr_analysis_info_t* info) {
+    if (info) free_info_arrays(info);
+    if (gpu_L && gpu_L->priv) free_mat(gpu_L);
+}
+
+int run_solver(sp_mat_t* gpu_L, dfr_analysis_info_t* info, const double* b, double* x, int n) {
+    if (!gpu_L || !gpu_L->priv) return fail(SPTRSV_E_INVALID_ARG, "matrix not allocated");
+    if (n != gpu_L->nr) return fail(SPTRSV_E_INVALID_ARG, "n differs from the matrix row count");
+    if (n == 0) return SPTRSV_OK;
+    if (!b || !x) return fail(SPTRSV_E_INVALID_ARG, "null b or x");
+    sptrsv_config_t c{SPTRSV_KERNEL_SIMPLE, 0, 0, 1};
+    dfr_analysis_info_t* use = nullptr;
+    if (info && info->mode != SIMPLE) {
+        use = info;
+        c.kernel = info->mode == ORDER ? SPTRSV_KERNEL_ORDER
+                   : info->mode == MULTIROW ? SPTRSV_KERNEL_MULTIROW : SPTRSV_KERNEL_AUTO;
+    }
+    double *db = nullptr, *dx = nullptr;
+    SPT_HIP(hipMalloc(&db, sizeof(double) * (size_t)n));
+    hipError_t e = hipMalloc(&dx, sizeof(double) * (size_t)n);
+    if (e != hipSuccess) {
+        (void)hipFree(db);
+        return hip_fail(e, "hipMalloc x", __FILE__, __LINE__);
+    }
+    int st = SPTRSV_OK;
+    if (hipMemcpy(db, b, sizeof(double) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        st = fail(SPTRSV_E_HIP, "copy b");
+    if (st == SPTRSV_OK) st = sptrsv_solve(gpu_L, use, SPTRSV_PREC_F64, &c, db, dx, n, 1, 1, nullptr);
+    if (st == SPTRSV_OK) st = sptrsv_check(gpu_L, nullptr);
+    if (st == SPTRSV_OK &&
+        hipMemcpy(x, dx, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+        st = fail(SPTRSV_E_HIP, "copy x");
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    return st;
+}
+
+// ---- device helpers -------------------------------------------------------------
+int sptrsv_device_count(int* count) {
+    if (!count) return fail(SPTRSV_E_INVALID_ARG, "null count");
+    SPT_HIP(hipGetDeviceCount(count));
+    return SPTRSV_OK;
+}
+
+int sptrsv_device_name(int device, char* buf, int len) {
+    if (!buf || len <= 0) return fail(SPTRSV_E_INVALID_ARG, "bad buffer");
+    hipDeviceProp_t p;
+    SPT_HIP(hipGetDeviceProperties(&p, device));
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return SPTRSV_OK;
+}
+
+int sptrsv_malloc(void** p, size_t bytes) {
+    if (!p) return fail(SPTRSV_E_INVALID_ARG, "null pointer");
+    SPT_HIP(hipMalloc(p, std::max<size_t>(bytes, 1)));
+    return SPTRSV_OK;
+}
+int sptrsv_free(void* p) {
+    if (p) SPT_HIP(hipFree(p));
+    return SPTRSV_OK;
+}
+int sptrsv_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    if (bytes) SPT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SPTRSV_OK;
+}
+int sptrsv_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    if (bytes) SPT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SPTRSV_OK;
+}
+int sptrsv_memset(void* dst, int value, size_t bytes) {
+    if (bytes) SPT_HIP(hipMemset(dst, value, bytes));
+    return SPTRSV_OK;
+}
+int sptrsv_synchronize(void) {
+    SPT_HIP(hipDeviceSynchronize());
+    return SPTRSV_OK;
+}
+
+int sptrsv_timer_start(hipStream_t stream) {
+    std::lock_guard<std::mutex> g(g_timer_mu);
+    if (!g_t0) {
+        SPT_HIP(hipEventCreate(&g_t0));
+        SPT_HIP(hipEventCreate(&g_t1));
+    }
+    SPT_HIP(hipEventRecord(g_t0, stream));
+    return SPTRSV_OK;
+}
+
+int sptrsv_timer_stop(hipStream_t stream, float* ms) {
+    std::lock_guard<std::mutex> g(g_timer_mu);
+    if (!g_t0 || !ms) return fail(SPTRSV_E_INVALID_ARG, "timer not started");
+    SPT_HIP(hipEventRecord(g_t1, stream));
+    SPT_HIP(hipEventSynchronize(g_t1));
+    SPT_HIP(hipEventElapsedTime(ms, g_t0, g_t1));
+    return SPTRSV_OK;
+}
+
+static __global__ void k_flush(uint4* p, size_t count, unsigned salt) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4((unsigned)i, salt, (unsigned)i ^ salt, salt + 1);
+}
+
+int sptrsv_flush_cache(hipStream_t stream) {
+    static uint4* buf = nullptr;
+    static unsigned salt = 0;
+    const size_t bytes = 512ull << 20;  // twice the 256 MB Infinity Cache
+    if (!buf) SPT_HIP(hipMalloc(&buf, bytes));
+    hipLaunchKernelGGL(k_flush, dim3(4096), dim3(256), 0, stream, buf, bytes / sizeof(uint4), ++salt);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+int sptrsv_measure_dep_latency(int rounds, float* one_way_ns) {
+    if (rounds < 1 || !one_way_ns) return fail(SPTRSV_E_INVALID_ARG, "bad arguments");
+    unsigned* flags = nullptr;
+    long long* out = nullptr;
+    SPT_HIP(hipMalloc(&flags, 64 * sizeof(unsigned)));
+    SPT_HIP(hipMalloc(&out, 4 * sizeof(long long)));
+    int dev = 0;
+    SPT_HIP(hipGetDevice(&dev));
+    int rate = 100000;
+    (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev);
+    std::vector<float> ns;
+    int st = SPTRSV_OK;
+    for (int partner = 1; partner < 8 && st == SPTRSV_OK; ++partner) {
+        SPT_HIP(hipMemset(flags, 0, 64 * sizeof(unsigned)));
+        SPT_HIP(hipMemset(out, 0, 4 * sizeof(long long)));
+        hipLaunchKernelGGL(k_pingpong, dim3(16), dim3(64), 0, nullptr, flags, partner, rounds, out, 1 << 22);
+        SPT_HIP(hipGetLastError());
+        long long h[4];
+        SPT_HIP(hipMemcpy(h, out, sizeof(h), hipMemcpyDeviceToHost));
+        if (h[2]) st = fail(SPTRSV_E_WATCHDOG, "ping-pong wait exceeded its bound");
+        else ns.push_back((float)(h[0] * (1e6 / rate) / (2.0 * rounds)));
+    }
+    (void)hipFree(flags);
+    (void)hipFree(out);
+    if (st != SPTRSV_OK) return st;
+    std::sort(ns.begin(), ns.end());
+    *one_way_ns = ns[ns.size() / 2];
+    return SPTRSV_OK;
+}
+
+int sptrsv_bench_solve(sp_mat_t* mat, dfr_analysis_info_t* info, int precision,
+                       const sptrsv_config_t* cfg, const void* b, void* x, int n, int k,
+                       int refine_steps, int reps, int check_ones, sptrsv_bench_t* out) {
+    if (!out || reps < 1) return fail(SPTRSV_E_INVALID_ARG, "bad bench arguments");
+    // warm-up (allocates workspaces outside the timed loop)
+    SPT_TRY(sptrsv_solve(mat, info, precision, cfg, b, x, n, k, refine_steps, nullptr));
+    SPT_HIP(hipDeviceSynchronize());
+    std::vector<hipEvent_t> ev(2 * reps);
+    for (auto& e : ev) SPT_HIP(hipEventCreate(&e));
+    int pw = amdsmiAPIRun("sptrsv_bench_solve", priv_of(mat)->device, 1000);
+    int st = SPTRSV_OK;
+    for (int r = 0; r < reps && st == SPTRSV_OK; ++r) {
+        SPT_HIP(hipEventRecord(ev[2 * r], nullptr));
+        st = sptrsv_solve(mat, info, precision, cfg, b, x, n, k, refine_steps, nullptr);
+        SPT_HIP(hipEventRecord(ev[2 * r + 1], nullptr));
+    }
+    SPT_HIP(hipDeviceSynchronize());
+    float power = pw == SPTRSV_OK ? amdsmiAPIEnd() : -1.0f;
+    if (pw == SPTRSV_OK && amdsmiAPISamples() == 0) power = -1.0f;
+    if (st != SPTRSV_OK) return st;
+    // mean and stdev (BENCH_RUN_SOLVE keeps a running mean of squares, common.h:112-118)
+    double sum = 0, sq = 0, mn = 1e30;
+    for (int r = 0; r < reps; ++r) {
+        float ms = 0;
+        SPT_HIP(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
+        sum += ms;
+        sq += (double)ms * ms;
+        mn = std::min<double>(mn, ms);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    double mean = sum / reps;
+    double var = sq / reps - mean * mean;
+    out->mean_ms = (float)mean;
+    out->stdev_ms = (float)sqrt(var > 0 ? var : 0);
+    out->min_ms = (float)mn;
+    out->power_mw = power;
+    out->reps = reps;
+    out->errors = -1;
+    SPT_TRY(sptrsv_check(mat, nullptr));
+    if (check_ones) {
+        size_t cnt = (size_t)n * k;
+        int errs = 0;
+        if (precision == SPTRSV_PREC_F32) {
+            std::vector<float> h(cnt);
+            SPT_HIP(hipMemcpy(h.data(), x, cnt * sizeof(float), hipMemcpyDeviceToHost));
+            for (float v : h) errs += (fabsf(1.0f - v) != 0.0f);
+        } else {
+            std::vector<double> h(cnt);
+            SPT_HIP(hipMemcpy(h.data(), x, cnt * sizeof(double), hipMemcpyDeviceToHost));
+            for (double v : h) errs += (fabs(1.0 - v) != 0.0);
+        }
+        out->errors = errs;
+    }
+    return SPTRSV_OK;
+}
+
+}  // extern "C"
