@@ -57,6 +57,15 @@ def bytes_method_convention(n, nnz, k):
     return 2 * f32 + res + upd
 
 
+KERNEL_NAMES = {1: "simple", 2: "order", 3: "multirow"}
+
+
+def cfg_dict(c) -> dict:
+    return {"kernel": KERNEL_NAMES.get(c.kernel, c.kernel), "lanes": c.lanes, "wg": c.wg,
+            "fused": c.fused, "variant": "one-row-per-wave" if c.variant else "multi-row-wave",
+            "cu_waves": c.cu_waves}
+
+
 def cpu_baseline(s, budget_s: float):
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as O
@@ -111,22 +120,21 @@ def main() -> int:
             tuned[prec] = I.default_config(mat, info, prec, k)
         else:
             tuned[prec], _ = I.autotune(mat, info, prec, k=k, reps=3)
-        c = tuned[prec]
-        log(f"[bench] precision {prec}: kernel={c.kernel} lanes={c.lanes} wg={c.wg}")
+        log(f"[bench] precision {prec}: {cfg_dict(tuned[prec])}")
 
     x_ref = O.fwd_f64(s.ia, s.ja, s.a, s.b, k=k)
 
     def run(prec):
         c = tuned[prec]
         for _ in range(args.warmup):
-            I.solve(mat, info, prec, db, dx, k=k, kernel=c.kernel, lanes=c.lanes, wg=c.wg, fused=1)
+            I.solve(mat, info, prec, db, dx, k=k, cfg=c)
         I.synchronize()
         assert I.check_watchdog(mat) == 0
         I.kernel_timer(True)
         timer = I.Timer()
         timer.start()
         for _ in range(args.steps):
-            I.solve(mat, info, prec, db, dx, k=k, kernel=c.kernel, lanes=c.lanes, wg=c.wg, fused=1)
+            I.solve(mat, info, prec, db, dx, k=k, cfg=c)
         ms = timer.stop() / args.steps
         kms, launches = I.kernel_timer_read()
         I.kernel_timer(False)
@@ -145,7 +153,7 @@ def main() -> int:
             I.synchronize()
             timer = I.Timer()
             timer.start()
-            I.solve(mat, info, prec, db, dx, k=k, kernel=c.kernel, lanes=c.lanes, wg=c.wg, fused=1)
+            I.solve(mat, info, prec, db, dx, k=k, cfg=c)
             out.append(timer.stop())
         return float(np.median(out))
 
@@ -157,11 +165,32 @@ def main() -> int:
     tdep = I.dep_latency_ns(2000)
     log(f"[bench] cold-cache: FP64 {cold64:.3f} ms, mixed {coldm:.3f} ms; t_dep {tdep:.0f} ns one-way")
 
+    # vendor baseline on the same device matrix (analysis timed separately)
+    rocsparse = None
+    try:
+        rb = I.RocsparseBaseline(mat)
+        rb.solve(db, dx)
+        I.synchronize()
+        timer = I.Timer()
+        timer.start()
+        for _ in range(args.steps):
+            rb.solve(db, dx)
+        rs_ms = timer.stop() / args.steps
+        x = dx.numpy()
+        rs_err = float(np.max(np.abs(x - x_ref)) / np.max(np.abs(x_ref)))
+        rocsparse = {"ms": round(rs_ms, 4), "gnnz_s": round(k * s.nnz / (rs_ms * 1e-3) / 1e9 if k == 1 else
+                                                           s.nnz / (rs_ms * 1e-3) / 1e9, 4),
+                     "analysis_ms": round(rb.analysis_ms, 3), "rel_err": rs_err,
+                     "note": "rocsparse_dcsrsv_solve, 1 RHS" if k == 1 else "rocsparse_dcsrsv_solve on RHS 0 only"}
+        rb.close()
+        log(f"[bench] rocSPARSE csrsv: {rs_ms:.3f} ms (analysis {rb.analysis_ms:.1f} ms)")
+    except Exception as e:  # reported baseline only
+        rocsparse = {"error": str(e)[:200]}
+
     power = None
     if I.power_start("bench", 0, 1000):
         for _ in range(max(args.steps, 50)):
-            I.solve(mat, info, I.PREC_MIXED, db, dx, k=k, kernel=tuned[I.PREC_MIXED].kernel,
-                    lanes=tuned[I.PREC_MIXED].lanes, wg=tuned[I.PREC_MIXED].wg, fused=1)
+            I.solve(mat, info, I.PREC_MIXED, db, dx, k=k, cfg=tuned[I.PREC_MIXED])
         I.synchronize()
         p = I.power_end()
         power = round(p / 1000.0, 1) if p > 0 else None
@@ -187,10 +216,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": desc, "n": n, "nnz": nnz, "rhs": k, "nlevs": nlevs,
-            "kernel_mixed": {"kernel": tuned[I.PREC_MIXED].kernel, "lanes": tuned[I.PREC_MIXED].lanes,
-                             "wg": tuned[I.PREC_MIXED].wg, "fused": 1},
-            "kernel_f64": {"kernel": tuned[I.PREC_F64].kernel, "lanes": tuned[I.PREC_F64].lanes,
-                           "wg": tuned[I.PREC_F64].wg},
+            "kernel_mixed": cfg_dict(tuned[I.PREC_MIXED]),
+            "kernel_f64": cfg_dict(tuned[I.PREC_F64]),
             "rel_err_mixed": errm, "rel_err_f64": err64,
             "plain_f64_gnnz_s": round(gnnz_64, 4), "plain_f64_ms": round(ms64, 4),
             "speedup_mixed_vs_f64": round(ms64 / msm, 3),
@@ -201,6 +228,7 @@ def main() -> int:
             "t_dep_ns": round(tdep, 1),
             "latency_bound_fraction_mixed": round(nlevs * tdep * 1e-6 / msm, 3),
             "power_w": power,
+            "rocsparse_csrsv": rocsparse,
         },
         "roofline": {
             "bound": "hbm",

@@ -168,12 +168,21 @@ int csr_L_solve_multirow_f32(sp_mat_t* mat, dfr_analysis_info_t* info, const flo
  * for SIMPLE/ORDER (1..64, power of two; 0 = from nnz/n as the reference
  * does, solve_csr_simple.cu:121-137); wg = threads per workgroup (256 or
  * 1024, 0 = 256); fused = 1 runs the refinement in the same sync-free pass as
- * the bulk solve, 0 runs separate passes (MIXED only). */
+ * the bulk solve, 0 runs separate passes (MIXED only); variant = 0 packs
+ * 64/lanes rows per wave (rows of a wave may depend on each other, natural
+ * order shares results through LDS), 1 is the lean path where no wave holds
+ * two dependent rows (SIMPLE/ORDER: one row per wave; MULTIROW: its
+ * single-level waves), one right-hand side only; cu_waves caps the waves
+ * resident per CU (0 = hardware limit): every resident wave polls, and past a
+ * few thousand polling waves the dependency hand-off latency grows several
+ * fold, so deep dependency chains run fastest with a small cap. */
 typedef struct {
     int kernel;
     int lanes;
     int wg;
     int fused;
+    int variant;
+    int cu_waves;
 } sptrsv_config_t;
 
 /* Generic solve with k right-hand sides (row-major n x k, x[i*k + r]).
@@ -198,6 +207,23 @@ int sptrsv_reserve(sp_mat_t* mat, int k);
  * 20000, environment SPTRSV_WATCHDOG_MS overrides. */
 int sptrsv_check(sp_mat_t* mat, hipStream_t stream);
 int sptrsv_set_watchdog_ms(sp_mat_t* mat, double ms);
+
+/* Polling back-off of the sync-free waits: 0 spins (default), 1..4 sleep
+ * about 64, 128, 256, 512 cycles between polling turns without progress. */
+int sptrsv_set_backoff(sp_mat_t* mat, int level);
+
+/* Extra LDS bytes (0..65536) reserved by every solve workgroup, which caps
+ * the workgroups resident per CU (160 KB of LDS each): fewer waves polling
+ * ahead of the dependency front.  0 (default) = no cap. */
+int sptrsv_set_lds_pad(sp_mat_t* mat, int bytes);
+
+/* Per-row timeline of the next solves on this matrix (profiling aid): while
+ * enabled, each solve writes three wall-clock stamps per row and right-hand
+ * side -- wave start, dependencies folded and reduced, result stored -- to a
+ * device buffer of 3*n*k int64.  _read copies count entries to host (ticks of
+ * the device wall clock; rate in kHz returned in *rate_khz). */
+int sptrsv_trace(sp_mat_t* mat, int k, int enable);
+int sptrsv_trace_read(sp_mat_t* mat, int64_t* host, size_t count, double* rate_khz);
 
 /* Times every candidate configuration on this matrix (and the given
  * precision) and stores the fastest in the matrix, where AUTO finds it.

@@ -57,7 +57,8 @@ class AnalysisInfo(Structure):  # dfr_analysis_info_t, dfr_syncfree.h:25-41
 
 
 class Config(Structure):  # sptrsv_config_t
-    _fields_ = [("kernel", c_int), ("lanes", c_int), ("wg", c_int), ("fused", c_int)]
+    _fields_ = [("kernel", c_int), ("lanes", c_int), ("wg", c_int), ("fused", c_int), ("variant", c_int),
+                ("cu_waves", c_int)]
 
 
 class BenchResult(Structure):  # sptrsv_bench_t
@@ -136,6 +137,10 @@ _PROTOS = {
     "sptrsv_kernel_timer": (c_int, [c_int]),
     "sptrsv_kernel_timer_read": (c_int, [POINTER(c_float), _P_INT]),
     "sptrsv_measure_dep_latency": (c_int, [c_int, POINTER(c_float)]),
+    "sptrsv_set_backoff": (c_int, [POINTER(SpMat), c_int]),
+    "sptrsv_set_lds_pad": (c_int, [POINTER(SpMat), c_int]),
+    "sptrsv_trace": (c_int, [POINTER(SpMat), c_int, c_int]),
+    "sptrsv_trace_read": (c_int, [POINTER(SpMat), POINTER(c_int64), c_size_t, POINTER(c_double)]),
 }
 
 EXPORTED = tuple(_PROTOS)

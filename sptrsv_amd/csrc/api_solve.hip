@@ -103,20 +103,36 @@ static bool has_waves(const dfr_analysis_info_t* info) {
 // Entry window: 4 when rows are long compared with the lanes walking them.
 static int window_for(const MatPriv* pv, int lanes) { return (pv->max_row > 2 * lanes) ? 4 : 1; }
 
+// Shape used for a kernel when only the kernel is named: one right-hand side
+// takes the lean path (one row per wave for SIMPLE/ORDER, with lanes covering
+// a typical row: the next power of two >= nnz/n, 4..64); several take the
+// general engine with the reference's lanes rule.
+static sptrsv_config_t shape_for(const sp_mat_t* m, int kernel, int k) {
+    sptrsv_config_t c{};
+    c.kernel = kernel;
+    c.wg = 256;
+    c.fused = 1;
+    const int K = pow2_ceil(k);
+    if (K == 1) {
+        c.variant = 1;
+        int avg = m->nr > 0 ? m->nnz / m->nr : 1;
+        c.lanes = std::min(kWave, std::max(4, pow2_ceil(avg)));
+    } else {
+        c.variant = 0;
+        int lanes = reference_lanes(m);
+        while (lanes * K > kWave && lanes > 1) lanes >>= 1;
+        c.lanes = lanes;
+    }
+    return c;
+}
+
 int default_config(const sp_mat_t* m, const dfr_analysis_info_t* info, int precision, int k,
                    sptrsv_config_t* out) {
     (void)precision;
-    sptrsv_config_t c{};
-    c.kernel = SPTRSV_KERNEL_SIMPLE;
-    int K = pow2_ceil(k);
-    int lanes = reference_lanes(m);
-    while (lanes * K > kWave && lanes > 1) lanes >>= 1;
-    c.lanes = lanes;
-    c.wg = 256;
-    c.fused = 1;
-    if (info && info->mode == ORDER && has_order(info)) c.kernel = SPTRSV_KERNEL_ORDER;
-    if (info && info->mode == MULTIROW && has_waves(info)) c.kernel = SPTRSV_KERNEL_MULTIROW;
-    *out = c;
+    int kernel = SPTRSV_KERNEL_SIMPLE;
+    if (info && info->mode == ORDER && has_order(info)) kernel = SPTRSV_KERNEL_ORDER;
+    if (info && info->mode == MULTIROW && has_waves(info)) kernel = SPTRSV_KERNEL_MULTIROW;
+    *out = shape_for(m, kernel, k);
     return SPTRSV_OK;
 }
 
@@ -125,6 +141,15 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     MatPriv* pv = priv_of(m);
     if (cfg && cfg->kernel != SPTRSV_KERNEL_AUTO) {
         *out = *cfg;
+        if (cfg->lanes <= 0) {
+            // only the kernel is named: the tuned shape for it, else the default shape
+            const sptrsv_config_t& t = pv->tuned_cfg[precision];
+            sptrsv_config_t d = (pv->tuned[precision] && pv->tuned_k[precision] == k && t.kernel == cfg->kernel)
+                                    ? t : shape_for(m, cfg->kernel, k);
+            out->lanes = d.lanes;
+            out->variant = d.variant;
+            if (cfg->wg <= 0) out->wg = d.wg;
+        }
     } else if (pv->tuned[precision] && pv->tuned_k[precision] == k &&
                (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_SIMPLE ||
                 (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_ORDER && has_order(info)) ||
@@ -141,8 +166,12 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
         default_config(m, info, precision, k, &d);
         out->lanes = d.lanes;
     }
-    if (out->wg != 256 && out->wg != 1024) out->wg = 256;
-    if (K > 1) out->wg = 256;
+    if (out->wg != 64 && out->wg != 256 && out->wg != 1024) out->wg = 256;
+    if (K > 1) {
+        out->wg = 256;
+        out->variant = 0;
+    }
+    if (out->cu_waves < 0) out->cu_waves = 0;
     if (out->lanes > kWave / K) out->lanes = kWave / K;
     if ((out->lanes & (out->lanes - 1)) != 0) return fail(SPTRSV_E_INVALID_ARG, "lanes must be a power of two");
     if (out->kernel == SPTRSV_KERNEL_ORDER && !has_order(info))
@@ -177,6 +206,33 @@ static int reserve(sp_mat_t* m, int k, bool need_pub, bool need_two_pass) {
         pv->resid_len = words;
     }
     return SPTRSV_OK;
+}
+
+// ---- residency cap ---------------------------------------------------------------------
+// Dynamic LDS that lets at most cu_waves waves of wg-thread workgroups share
+// a CU (160 KB of LDS per CU on gfx950; at most 64 KB per workgroup, so the
+// floor is two workgroups per CU).  The kernels do not touch this LDS.
+constexpr unsigned kLdsPerCu = 160u * 1024u;
+constexpr unsigned kStaticLdsSlack = 8u * 1024u + 512u;  // largest static LDS of a solve kernel
+
+static int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+unsigned lds_pad_for(int cu_waves, int wg) {
+    if (cu_waves <= 0) return 0;
+    const int waves_per_wg = std::max(1, wg / kWave);
+    const int wgs = std::max(1, cu_waves / waves_per_wg);
+    long long pad = (long long)(kLdsPerCu / (unsigned)wgs) - (long long)kStaticLdsSlack;
+    if (pad <= 0) return 0;
+    return (unsigned)std::min<long long>(pad, 65536);
 }
 
 // ---- kernel timer ---------------------------------------------------------------------
@@ -227,6 +283,9 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     A.ticket = pv->counters;
     A.err = pv->counters + 1;
     A.wd_ticks = pv->watchdog_ticks;
+    A.trace = (pv->trace && pv->trace_len >= 3 * (size_t)m->nr * k) ? pv->trace : nullptr;
+    A.backoff = pv->backoff;
+    A.dyn_lds = pv->lds_pad ? pv->lds_pad : lds_pad_for(c.cu_waves, c.wg);
     size_t words = (size_t)m->nr * k;
     using word = typename P::word;
     hipLaunchKernelGGL((k_prepare<word>), dim3(stream_grid(words)), dim3(256), 0, s,
@@ -234,7 +293,15 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     SPT_HIP(hipGetLastError());
     int kind = c.kernel == SPTRSV_KERNEL_ORDER ? KK_ORDER
                : c.kernel == SPTRSV_KERNEL_MULTIROW ? KK_MULTIROW : KK_SIMPLE;
-    int u = window_for(pv, kind == KK_MULTIROW ? 32 : c.lanes);
+    if (c.variant == 1 && K == 1) kind += KK_FAST_SIMPLE - KK_SIMPLE;
+    if (c.variant == 2 && K == 1) {
+        kind += KK_PERSIST_SIMPLE - KK_SIMPLE;
+        // persistent: the wave count itself is the residency cap
+        A.dyn_lds = pv->lds_pad;
+        const int per_cu = c.cu_waves > 0 ? c.cu_waves : 4;
+        A.persist_grid = (long long)device_cus() * std::max(1, per_cu / std::max(1, c.wg / kWave));
+    }
+    int u = window_for(pv, (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW) ? 32 : c.lanes);
     hipEvent_t* ev = kernel_timer_pair(s);
     if (ev) SPT_HIP(hipEventRecord(ev[0], s));
     hipError_t e = launch_solve<P, 64>(kind, K, c.wg, u, A, s);
@@ -409,6 +476,43 @@ int sptrsv_check(sp_mat_t* mat, hipStream_t stream) {
     return SPTRSV_OK;
 }
 
+int sptrsv_set_backoff(sp_mat_t* mat, int level) {
+    if (!mat || !mat->priv || level < 0 || level > 4) return fail(SPTRSV_E_INVALID_ARG, "bad back-off");
+    priv_of(mat)->backoff = level;
+    return SPTRSV_OK;
+}
+
+int sptrsv_set_lds_pad(sp_mat_t* mat, int bytes) {
+    if (!mat || !mat->priv || bytes < 0 || bytes > 65536) return fail(SPTRSV_E_INVALID_ARG, "bad LDS pad");
+    priv_of(mat)->lds_pad = (unsigned)bytes;
+    return SPTRSV_OK;
+}
+
+int sptrsv_trace(sp_mat_t* mat, int k, int enable) {
+    if (!mat || !mat->priv || k < 1 || k > kWave) return fail(SPTRSV_E_INVALID_ARG, "bad trace request");
+    MatPriv* pv = priv_of(mat);
+    if (pv->trace) SPT_HIP(hipFree(pv->trace));
+    pv->trace = nullptr;
+    pv->trace_len = 0;
+    if (enable) {
+        size_t len = 3 * (size_t)mat->nr * k;
+        SPT_HIP(hipMalloc(&pv->trace, std::max<size_t>(len, 1) * sizeof(long long)));
+        SPT_HIP(hipMemset(pv->trace, 0, std::max<size_t>(len, 1) * sizeof(long long)));
+        pv->trace_len = len;
+    }
+    return SPTRSV_OK;
+}
+
+int sptrsv_trace_read(sp_mat_t* mat, int64_t* host, size_t count, double* rate_khz) {
+    if (!mat || !mat->priv || !host) return fail(SPTRSV_E_INVALID_ARG, "bad trace read");
+    MatPriv* pv = priv_of(mat);
+    if (!pv->trace || count > pv->trace_len) return fail(SPTRSV_E_INVALID_ARG, "trace not enabled or too short");
+    SPT_HIP(hipDeviceSynchronize());
+    SPT_HIP(hipMemcpy(host, pv->trace, count * sizeof(long long), hipMemcpyDeviceToHost));
+    if (rate_khz) *rate_khz = pv->wall_rate_khz;
+    return SPTRSV_OK;
+}
+
 int sptrsv_kernel_timer(int enable) {
     std::lock_guard<std::mutex> g(g_kt_mu);
     g_kt_on = enable != 0;
@@ -455,38 +559,91 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
     SPT_HIP(hipMemset(b, 0, std::max<size_t>(words, 1) * esz));  // any finite right-hand side
     if (precision == SPTRSV_PREC_MIXED) SPT_TRY(reserve(mat, k, true, true));
 
-    std::vector<sptrsv_config_t> cand;
+    // Shapes: (kernel, lanes, workgroup, variant).
+    std::vector<sptrsv_config_t> shapes;
     int ref = std::min(reference_lanes(mat), kWave / K);
     int lo = std::max(1, ref / 2), hi = std::min(kWave / K, ref * 4);
     std::vector<int> wgs = (K == 1) ? std::vector<int>{256, 1024} : std::vector<int>{256};
+    // general engine (several rows per wave)
     for (int L = lo; L <= hi; L <<= 1)
-        for (int wg : wgs) cand.push_back({SPTRSV_KERNEL_SIMPLE, L, wg, 1});
+        for (int wg : wgs) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, wg, 1, 0, 0});
     if (has_order(info))
-        for (int L = lo; L <= hi; L <<= 1) cand.push_back({SPTRSV_KERNEL_ORDER, L, 256, 1});
+        for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, 256, 1, 0, 0});
     if (has_waves(info))
-        for (int wg : wgs) cand.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1});
+        for (int wg : wgs) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1, 0, 0});
+    // lean path (one right-hand side): one row per wave, or single-level waves
+    if (K == 1) {
+        const int flo = std::max(4, ref), fhi = std::min(kWave, ref * 8);
+        for (int L = flo; L <= fhi; L <<= 1)
+            for (int wg : {64, 256}) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, wg, 1, 1, 0});
+        if (has_order(info))
+            for (int L = flo; L <= fhi; L <<= 1)
+                for (int wg : {64, 256}) shapes.push_back({SPTRSV_KERNEL_ORDER, L, wg, 1, 1, 0});
+        if (has_waves(info))
+            for (int wg : {64, 256}) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1, 1, 0});
+        // persistent lean path: a fixed number of resident waves walks the rows
+        for (int L = flo; L <= fhi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, 64, 1, 2, 0});
+        if (has_order(info))
+            for (int L = flo; L <= fhi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, 64, 1, 2, 0});
+        if (has_waves(info)) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, 64, 1, 2, 0});
+    }
 
     hipEvent_t e0, e1;
     SPT_HIP(hipEventCreate(&e0));
     SPT_HIP(hipEventCreate(&e1));
-    float best_t = 1e30f;
-    sptrsv_config_t bc = cand.front();
     int status = SPTRSV_OK;
-    for (const auto& c : cand) {
+    auto time_cfg = [&](const sptrsv_config_t& c) -> float {
         // one warm-up, then reps timed solves
         status = solve_impl(mat, info, precision, c, b, x, k, 1, nullptr);
-        if (status != SPTRSV_OK) break;
-        SPT_HIP(hipEventRecord(e0, nullptr));
+        if (status != SPTRSV_OK) return 1e30f;
+        if (hipEventRecord(e0, nullptr) != hipSuccess) return 1e30f;
         for (int r = 0; r < reps && status == SPTRSV_OK; ++r)
             status = solve_impl(mat, info, precision, c, b, x, k, 1, nullptr);
-        SPT_HIP(hipEventRecord(e1, nullptr));
-        SPT_HIP(hipEventSynchronize(e1));
+        if (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return 1e30f;
         float ms = 0;
-        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= reps;
-        if (ms < best_t) {
-            best_t = ms;
-            bc = c;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return 1e30f;
+        return ms / reps;
+    };
+    // Stage 1: every shape without and with a tight residency cap.
+    std::vector<std::pair<float, sptrsv_config_t>> ranked;
+    for (auto c : shapes) {
+        for (int cw : (c.variant == 2 ? std::vector<int>{2, 8} : std::vector<int>{0, 4})) {
+            c.cu_waves = cw;
+            float t = time_cfg(c);
+            if (status != SPTRSV_OK) break;
+            ranked.push_back({t, c});
+        }
+        if (status != SPTRSV_OK) break;
+    }
+    // Stage 2: the four best shapes across residency caps.
+    float best_t = 1e30f;
+    sptrsv_config_t bc = shapes.front();
+    if (status == SPTRSV_OK) {
+        std::sort(ranked.begin(), ranked.end(),
+                  [](const auto& a, const auto& b) { return a.first < b.first; });
+        best_t = ranked.front().first;
+        bc = ranked.front().second;
+        int taken = 0;
+        for (size_t i = 0; i < ranked.size() && taken < 4 && status == SPTRSV_OK; ++i) {
+            sptrsv_config_t c = ranked[i].second;
+            bool dup = false;
+            for (size_t j = 0; j < i; ++j) {
+                const auto& d = ranked[j].second;
+                dup |= d.kernel == c.kernel && d.lanes == c.lanes && d.wg == c.wg && d.variant == c.variant;
+            }
+            if (dup) continue;
+            ++taken;
+            for (int cw : (c.variant == 2 ? std::vector<int>{1, 3, 4, 6, 12, 16}
+                                          : std::vector<int>{32, 16, 8, 2})) {
+                c.cu_waves = cw;
+                float t = time_cfg(c);
+                if (status != SPTRSV_OK) break;
+                if (t < best_t) {
+                    best_t = t;
+                    bc = c;
+                }
+            }
         }
     }
     hipEventDestroy(e0);

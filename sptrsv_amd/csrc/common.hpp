@@ -50,6 +50,10 @@ struct MatPriv {
     double wall_rate_khz = 100000.0;
     int device = 0;
     int max_row = 0;             // longest row (entries incl. diagonal)
+    long long* trace = nullptr;  // per-row timeline (profiling), 3*n*k
+    size_t trace_len = 0;
+    int backoff = 0;
+    unsigned lds_pad = 0;        // extra LDS per workgroup, caps resident waves
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};
     int tuned_k[3] = {0, 0, 0};

@@ -77,6 +77,7 @@ static void free_mat(sp_mat_t* m) {
         if (pv->b32) (void)hipFree(pv->b32);
         if (pv->corr) (void)hipFree(pv->corr);
         if (pv->resid) (void)hipFree(pv->resid);
+        if (pv->trace) (void)hipFree(pv->trace);
         delete pv;
     }
     memset(m, 0, sizeof(*m));
@@ -89,6 +90,8 @@ static int validate_host_csr(const int* ia, const int* ja, int n, int* max_row) 
     if (n == 0) { *max_row = 0; return SPTRSV_OK; }
     if (!ia || !ja) return fail(SPTRSV_E_INVALID_ARG, "null CSR arrays");
     const int base = ia[0];
+    if (base < 0) return fail(SPTRSV_E_INVALID_CSR, "negative row pointer");
+    ja += base;  // rows index ja relative to ia[0]
     int mr = 0;
     for (int i = 0; i < n; ++i) {
         long long s = (long long)ia[i] - base, e = (long long)ia[i + 1] - base;

@@ -32,6 +32,8 @@
 //    malformed input ends the kernel instead of hanging the GPU.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace sptrsv {
@@ -55,6 +57,10 @@ struct SolveArgs {
     int* ticket;
     int* err;
     long long wd_ticks;
+    long long* trace;       // optional per-row timeline: start, reduced, stored (wall clock)
+    int backoff;            // 0: spin; 1..4: s_sleep between polling turns without progress
+    unsigned dyn_lds;       // extra LDS bytes per workgroup (caps resident waves; unused by the kernels)
+    long long persist_grid; // persistent kernels: workgroups to launch
 };
 
 template <int K>
@@ -89,6 +95,70 @@ __device__ __forceinline__ uint32_t canon32(float v) {
     return u == kUnsolved32 ? 0x7fc00000u : u;
 }
 
+// ---- cross-lane reduction steps (one lane group of G lanes, K == 1) ----------
+// Step s combines lanes 2^s apart: s = 0, 1 swap within quads (DPP quad_perm),
+// s = 2, 3 mirror within half-rows / rows of 16 (DPP row_half_mirror,
+// row_mirror; each half already holds one value after the earlier steps, so a
+// mirror pairs the two halves), s = 4, 5 exchange rows 16 / 32 lanes apart
+// (v_permlane16_swap / v_permlane32_swap).  Every lane of the group ends with
+// the same, bit-identical total (the combine is commutative).
+__device__ __forceinline__ uint32_t dpp_partner(uint32_t v, int s) {
+    switch (s) {
+        case 0: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+        case 1: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+        case 2: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);
+        default: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);
+    }
+}
+
+template <class T, class Op>
+__device__ __forceinline__ T lane_combine(T v, int s, Op op) {
+    constexpr int N = sizeof(T) / 4;
+    uint32_t w[N];
+    __builtin_memcpy(w, &v, sizeof(T));
+    uint32_t a[N], b[N];
+    if (s < 4) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            a[i] = w[i];
+            b[i] = dpp_partner(w[i], s);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            auto r = (s == 4) ? __builtin_amdgcn_permlane16_swap(w[i], w[i], false, false)
+                              : __builtin_amdgcn_permlane32_swap(w[i], w[i], false, false);
+            a[i] = r[0];
+            b[i] = r[1];
+        }
+    }
+    T x, y;
+    __builtin_memcpy(&x, a, sizeof(T));
+    __builtin_memcpy(&y, b, sizeof(T));
+    return op(x, y);
+}
+
+// Value of lane `src` (wave-uniform) in every lane, via v_readlane.
+template <class T>
+__device__ __forceinline__ T bcast(T v, int src) {
+    constexpr int N = (sizeof(T) + 3) / 4;
+    uint32_t w[N] = {};
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = (uint32_t)__builtin_amdgcn_readlane((int)w[i], src);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
+struct OpAdd {
+    template <class T>
+    __device__ T operator()(T x, T y) const { return x + y; }
+};
+struct OpMax {
+    __device__ int operator()(int x, int y) const { return max(x, y); }
+};
+
 // ---- precision / operation policies -----------------------------------------
 
 // Plain FP64: x_i = (b_i - sum L_ij x_j) * (1 / L_ii), published into x.
@@ -98,20 +168,25 @@ struct PolF64 {
     static constexpr word kPoison = 0x7ff8000000000000ull;
     struct Acc { double s; };
     struct Val { double v; };
+    struct Diag { double piv; };
     __device__ static Acc zero() { return {0.0}; }
     __device__ static Acc seed(const SolveArgs& A, size_t bi) {
         return {static_cast<const double*>(A.b)[bi]};
     }
     __device__ static Val load(const SolveArgs& A, int p) { return {A.a64[p]}; }
+    __device__ static Diag diag(const SolveArgs& A, int dpos) { return {1.0 / A.a64[dpos]}; }
     __device__ static void fold(Acc& c, const Val& v, word w) {
         c.s = __fma_rn(-v.v, __longlong_as_double((long long)w), c.s);
     }
+    struct Out {};
     __device__ static Acc add(const Acc& x, const Acc& y) { return {x.s + y.s}; }
     __device__ static Acc shfl_xor(const Acc& c, int o) { return {__shfl_xor(c.s, o)}; }
-    __device__ static word finish(const SolveArgs& A, int dpos, size_t xi, const Acc& t) {
-        (void)xi;
-        return canon64(t.s * (1.0 / A.a64[dpos]));
+    __device__ static Acc reduce_step(const Acc& c, int s) { return {lane_combine(c.s, s, OpAdd{})}; }
+    __device__ static word finish(const SolveArgs&, const Diag& d, const Acc& t, Out&) {
+        return canon64(t.s * d.piv);
     }
+    __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
+    __device__ static Out bcast_out(const Out& o, int) { return o; }
 };
 
 // Plain FP32 on the FP32 copy of the values, published into x (float).
@@ -121,20 +196,25 @@ struct PolF32 {
     static constexpr word kPoison = 0x7fc00000u;
     struct Acc { float s; };
     struct Val { float v; };
+    struct Diag { float piv; };
     __device__ static Acc zero() { return {0.0f}; }
     __device__ static Acc seed(const SolveArgs& A, size_t bi) {
         return {static_cast<const float*>(A.b)[bi]};
     }
     __device__ static Val load(const SolveArgs& A, int p) { return {A.a32[p]}; }
+    __device__ static Diag diag(const SolveArgs& A, int dpos) { return {1.0f / A.a32[dpos]}; }
     __device__ static void fold(Acc& c, const Val& v, word w) {
         c.s = __fmaf_rn(-v.v, __uint_as_float(w), c.s);
     }
+    struct Out {};
     __device__ static Acc add(const Acc& x, const Acc& y) { return {x.s + y.s}; }
     __device__ static Acc shfl_xor(const Acc& c, int o) { return {__shfl_xor(c.s, o)}; }
-    __device__ static word finish(const SolveArgs& A, int dpos, size_t xi, const Acc& t) {
-        (void)xi;
-        return canon32(t.s * (1.0f / A.a32[dpos]));
+    __device__ static Acc reduce_step(const Acc& c, int s) { return {lane_combine(c.s, s, OpAdd{})}; }
+    __device__ static word finish(const SolveArgs&, const Diag& d, const Acc& t, Out&) {
+        return canon32(t.s * d.piv);
     }
+    __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
+    __device__ static Out bcast_out(const Out& o, int) { return o; }
 };
 
 // Fused mixed precision, one refinement step in the same dependency pass:
@@ -151,12 +231,16 @@ struct PolMixed {
     static constexpr word kPoison = 0x7fc000007fc00000ull;
     struct Acc { float x0; float d; double r; };
     struct Val { float v32; double v64; };
+    struct Diag { float piv; double a; };
     __device__ static Acc zero() { return {0.0f, 0.0f, 0.0}; }
     __device__ static Acc seed(const SolveArgs& A, size_t bi) {
         double b = static_cast<const double*>(A.b)[bi];
         return {(float)b, 0.0f, b};
     }
     __device__ static Val load(const SolveArgs& A, int p) { return {A.a32[p], A.a64[p]}; }
+    __device__ static Diag diag(const SolveArgs& A, int dpos) {
+        return {1.0f / A.a32[dpos], A.a64[dpos]};
+    }
     __device__ static void fold(Acc& c, const Val& v, word w) {
         float x0 = __uint_as_float((uint32_t)w);
         float d = __uint_as_float((uint32_t)(w >> 32));
@@ -170,12 +254,19 @@ struct PolMixed {
     __device__ static Acc shfl_xor(const Acc& c, int o) {
         return {__shfl_xor(c.x0, o), __shfl_xor(c.d, o), __shfl_xor(c.r, o)};
     }
-    __device__ static word finish(const SolveArgs& A, int dpos, size_t xi, const Acc& t) {
-        float piv = 1.0f / A.a32[dpos];
+    __device__ static Acc reduce_step(const Acc& c, int s) {
+        return {lane_combine(c.x0, s, OpAdd{}), lane_combine(c.d, s, OpAdd{}),
+                lane_combine(c.r, s, OpAdd{})};
+    }
+    struct Out { double x; };
+    __device__ static void emit(const SolveArgs& A, size_t xi, const Out& o) { A.x_out[xi] = o.x; }
+    __device__ static Out bcast_out(const Out& o, int src) { return {bcast(o.x, src)}; }
+    __device__ static word finish(const SolveArgs&, const Diag& dg, const Acc& t, Out& o) {
+        const float piv = dg.piv;
         float x0 = t.x0 * piv;
-        double r = __fma_rn(-A.a64[dpos], (double)x0, t.r);
+        double r = __fma_rn(-dg.a, (double)x0, t.r);
         float d = ((float)r + t.d) * piv;
-        A.x_out[xi] = (double)x0 + (double)d;
+        o.x = (double)x0 + (double)d;
         uint32_t lo = canon32(x0);
         uint32_t hi = __float_as_uint(d);
         return ((uint64_t)hi << 32) | lo;  // never all-ones: lo is canonical
@@ -190,16 +281,35 @@ struct PolLevel {
     static constexpr word kPoison = 1u;
     struct Acc { int m; };
     struct Val {};
+    struct Diag {};
     __device__ static Acc zero() { return {0}; }
     __device__ static Acc seed(const SolveArgs&, size_t) { return {0}; }
     __device__ static Val load(const SolveArgs&, int) { return {}; }
+    __device__ static Diag diag(const SolveArgs&, int) { return {}; }
     __device__ static void fold(Acc& c, const Val&, word w) { c.m = max(c.m, (int)w); }
+    struct Out {};
     __device__ static Acc add(const Acc& x, const Acc& y) { return {max(x.m, y.m)}; }
     __device__ static Acc shfl_xor(const Acc& c, int o) { return {__shfl_xor(c.m, o)}; }
-    __device__ static word finish(const SolveArgs&, int, size_t, const Acc& t) {
+    __device__ static Acc reduce_step(const Acc& c, int s) { return {lane_combine(c.m, s, OpMax{})}; }
+    __device__ static word finish(const SolveArgs&, const Diag&, const Acc& t, Out&) {
         return (word)(1 + t.m);
     }
+    __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
+    __device__ static Out bcast_out(const Out& o, int) { return o; }
 };
+
+// All lanes of a group of G lanes (K right-hand-side lanes interleaved)
+// receive the group's total.
+template <class P, int K>
+__device__ __forceinline__ typename P::Acc group_reduce(typename P::Acc t, int G) {
+    if constexpr (K == 1) {
+        int s = 0;
+        for (int o = 1; o < G; o <<= 1, ++s) t = P::reduce_step(t, s);
+    } else {
+        for (int o = K; o < G; o <<= 1) t = P::add(t, P::shfl_xor(t, o));
+    }
+    return t;
+}
 
 // ---- the row engine -----------------------------------------------------------
 // One call handles the lane group of `row` (row < 0: idle group) for the
@@ -228,6 +338,10 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
     }
     const size_t xi = (size_t)(live ? row : 0) * A.k + r;
     Acc acc = (live && e == 0) ? P::seed(A, xi) : P::zero();
+    // The diagonal (and its reciprocal) is independent of the dependencies:
+    // load it now so it is off the critical path at publish time.
+    typename P::Diag dg{};
+    if (live && e == 0) dg = P::diag(A, dpos);
 
     int p = rs + e;     // first entry of the current window
     int kc = 0;         // entries of the window already folded
@@ -249,63 +363,119 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
     const unsigned long long gmask =
         (G == 64) ? ~0ull : (((1ull << G) - 1ull) << ((lane / G) * G));
     const long long t0 = wall_clock64();
+    if (A.trace && live && e == 0) A.trace[3 * xi] = t0;
     bool expired = false;
     int spins = 0;
+    // A reduced row waits one loop turn before it is stored: on gfx9 the
+    // vector-memory counter retires in order, so a poll issued after a store
+    // could not be consumed before that store completed.  Issuing the turn's
+    // global polls first and the stores after them keeps the store latency
+    // off every poll.
+    bool pending = false;
+    word pend_w = 0;
+    typename P::Out pend_o{};
 
     while (true) {
-        if (!done) {
-            word w[U];
+        bool moved = false;  // a dependency was folded this turn
+        // (1) global polls of the window entries not folded yet
+        word w[U];
+        bool in_lds[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                int q = p + u * L;
-                w[u] = P::kUnsolved;
-                if (u >= kc && q < dpos) {
-                    int c = col[u];
-                    if (c >= lds_lo && c < lds_hi)
-                        w[u] = poll_lds(&s_pub[c - lds_lo]);
-                    else
-                        w[u] = poll_global(&gpub[(size_t)c * A.k + r]);
-                    if (expired && w[u] == P::kUnsolved) w[u] = P::kPoison;
-                }
+        for (int u = 0; u < U; ++u) {
+            const int q = p + u * L;
+            w[u] = P::kUnsolved;
+            in_lds[u] = false;
+            if (!done && u >= kc && q < dpos) {
+                const int c = col[u];
+                if (c >= lds_lo && c < lds_hi)
+                    in_lds[u] = true;
+                else
+                    w[u] = poll_global(&gpub[(size_t)c * A.k + r]);
             }
+        }
+        asm volatile("" ::: "memory");
+        // (3) LDS polls, after this wave's own LDS stores; (4) fold the ready
+        // prefix of the window in storage order.  LDS results live in their
+        // own registers so they never wait on the global polls.
+        auto consume = [&]() {
+            word wl[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                int q = p + u * L;
-                if (u == kc && q < dpos && w[u] != P::kUnsolved) {
-                    P::fold(acc, val[u], w[u]);
-                    ++kc;
-                }
-            }
-            if (p + kc * L >= dpos) {
-                done = true;
-            } else if (kc == U) {
-                p += U * L;
-                kc = 0;
+            for (int u = 0; u < U; ++u)
+                wl[u] = in_lds[u] ? poll_lds(&s_pub[col[u] - lds_lo]) : P::kUnsolved;
+            if (!done) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    int q = p + u * L;
-                    if (q < dpos) {
-                        col[u] = A.ja[q];
-                        val[u] = P::load(A, q);
+                    const int q = p + u * L;
+                    word v = in_lds[u] ? wl[u] : w[u];
+                    if (expired && u >= kc && q < dpos && v == P::kUnsolved) v = P::kPoison;
+                    if (u == kc && q < dpos && v != P::kUnsolved) {
+                        P::fold(acc, val[u], v);
+                        moved = true;
+                        ++kc;
+                    }
+                }
+                if (p + kc * L >= dpos) {
+                    done = true;
+                } else if (kc == U) {
+                    p += U * L;
+                    kc = 0;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int q = p + u * L;
+                        if (q < dpos) {
+                            col[u] = A.ja[q];
+                            val[u] = P::load(A, q);
+                        }
                     }
                 }
             }
+        };
+        // (2) stores of the rows reduced in the previous turn.  The branch is
+        // wave-uniform and consume() is repeated in both arms, so each arm
+        // waits for exactly its own outstanding memory operations.
+        const unsigned long long pmask = __ballot(pending);
+        if (pmask) {
+            // Every lane stores: lanes without a row to publish repeat the
+            // first pending lane's store (same address, same value), so the
+            // store is not skipped on any path and the polls above are waited
+            // for with the stores still in flight.
+            const int src = __builtin_ctzll(pmask);
+            const size_t sxi = pending ? xi : bcast(xi, src);
+            const word sw = pending ? pend_w : bcast(pend_w, src);
+            const typename P::Out so = pending ? pend_o : P::bcast_out(pend_o, src);
+            if (pending && row >= lds_lo && row < lds_hi) publish_lds(&s_pub[row - lds_lo], pend_w);
+            if (A.trace && pending) A.trace[3 * xi + 2] = wall_clock64();
+            P::emit(A, sxi, so);
+            publish_global(&gpub[sxi], sw);
+            pending = false;
+            asm volatile("" ::: "memory");
+            consume();
+        } else {
+            consume();
         }
+        // (5) reduce the groups whose lanes are all done
         const unsigned long long dm = __ballot(done);
         const bool ready = ((dm & gmask) == gmask) && !published;
         if (__ballot(ready)) {
-            Acc t = acc;
-            for (int o = K; o < G; o <<= 1) t = P::add(t, P::shfl_xor(t, o));
+            const Acc t = group_reduce<P, K>(acc, G);
             if (ready) {
                 if (e == 0) {
-                    word wv = P::finish(A, dpos, xi, t);
-                    if (row >= lds_lo && row < lds_hi) publish_lds(&s_pub[row - lds_lo], wv);
-                    publish_global(&gpub[xi], wv);
+                    if (A.trace) A.trace[3 * xi + 1] = wall_clock64();
+                    pend_w = P::finish(A, dg, t, pend_o);
+                    pending = true;
                 }
                 published = true;
             }
         }
-        if (__ballot(!published) == 0ull) break;
+        if (__ballot(!published || pending) == 0ull) break;
+        if (A.backoff && !__ballot(moved || pending)) {
+            switch (A.backoff) {
+                case 1: __builtin_amdgcn_s_sleep(1); break;
+                case 2: __builtin_amdgcn_s_sleep(2); break;
+                case 3: __builtin_amdgcn_s_sleep(4); break;
+                default: __builtin_amdgcn_s_sleep(8); break;
+            }
+        }
         if (++spins == 512) {
             spins = 0;
             if (!expired && wall_clock64() - t0 > A.wd_ticks) {
@@ -383,6 +553,153 @@ __global__ __launch_bounds__(WG) void k_multirow(SolveArgs A) {
     }
 }
 
+// ---- fast path: no dependency inside a wave -----------------------------------
+// For layouts where the rows of one wave cannot depend on each other (one row
+// per wave, or a multirow wave whose rows share a level), a wave polls until
+// every dependency of its current chunk of entries is present, folds the chunk
+// in storage order, and reduces and publishes once at the end.  The polling
+// turn is a handful of instructions (predicated loads, compares, one ballot),
+// which keeps the detection delay close to the memory round trip.  Chunks are
+// U entries per lane; folding order per lane is the same as sf_rows'.
+template <class P, int U>
+__device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, int row) {
+    using word = typename P::word;
+    using Acc = typename P::Acc;
+    using Val = typename P::Val;
+    word* gpub = static_cast<word*>(A.pub);
+    const int e = lane & (L - 1);
+    const bool live = row >= 0;
+    int rs = 0, dpos = 0;
+    if (live) {
+        rs = A.ia[row];
+        dpos = A.ia[row + 1] - 1;
+    }
+    const size_t xi = live ? (size_t)row : 0;
+    Acc acc = (live && e == 0) ? P::seed(A, xi) : P::zero();
+    typename P::Diag dg{};
+    if (live && e == 0) dg = P::diag(A, dpos);
+    if (A.trace && live && e == 0) A.trace[3 * xi] = wall_clock64();
+    const long long t0 = wall_clock64();
+    bool expired = false;
+    int spins = 0;
+    for (int p = rs + e; __ballot(live && p < dpos); p += U * L) {
+        int col[U];
+        Val val[U];
+        word got[U];
+        bool pend[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = p + u * L;
+            pend[u] = live && q < dpos;
+            col[u] = 0;
+            got[u] = P::kUnsolved;
+            if (pend[u]) {
+                col[u] = A.ja[q];
+                val[u] = P::load(A, q);
+            }
+        }
+        while (true) {
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (pend[u]) {
+                    word w = poll_global(&gpub[col[u]]);
+                    if (w != P::kUnsolved) {
+                        got[u] = w;
+                        pend[u] = false;
+                    } else if (expired) {
+                        got[u] = P::kPoison;
+                        pend[u] = false;
+                    }
+                    any |= pend[u];
+                }
+            }
+            if (!__ballot(any)) break;
+            if (A.backoff) __builtin_amdgcn_s_sleep(1);
+            if (++spins == 512) {
+                spins = 0;
+                if (!expired && wall_clock64() - t0 > A.wd_ticks) {
+                    expired = true;
+                    if (lane == 0) atomicOr(A.err, 1);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (live && p + u * L < dpos) P::fold(acc, val[u], got[u]);
+    }
+    const Acc t = group_reduce<P, 1>(acc, L);
+    if (live && e == 0) {
+        if (A.trace) A.trace[3 * xi + 1] = wall_clock64();
+        typename P::Out o{};
+        const word w = P::finish(A, dg, t, o);
+        P::emit(A, xi, o);
+        if (A.trace) A.trace[3 * xi + 2] = wall_clock64();
+        publish_global(&gpub[xi], w);
+    }
+}
+
+// One row per wave, natural (ORDERED = false) or level order (true).
+template <class P, int WG, int U, bool ORDERED>
+__global__ __launch_bounds__(WG) void k_fast_row(SolveArgs A) {
+    __shared__ int s_base;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_base = atomicAdd(A.ticket, 1) * (WG / kWave);
+    __syncthreads();
+    const int pos = s_base + (tid >> 6);
+    if (pos >= A.n) return;  // whole wave
+    const int row = ORDERED ? A.iorder[pos] : pos;
+    const int lane = tid & (kWave - 1);
+    fast_rows<P, U>(A, A.L, lane, lane < A.L ? row : -1);
+}
+
+// Multirow waves (rows of one wave share a level).
+template <class P, int WG, int U>
+__global__ __launch_bounds__(WG) void k_fast_multirow(SolveArgs A) {
+    __shared__ int s_t;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_t = atomicAdd(A.ticket, 1);
+    __syncthreads();
+    const int w = s_t * (WG / kWave) + (tid >> 6);
+    if (w >= A.n_waves) return;
+    const int lane = tid & (kWave - 1);
+    const int vs = A.wsize[w];
+    const int L = vs == 0 ? 1 : vs;
+    const int b0 = A.wbase[w];
+    const int nrows = A.wbase[w + 1] - b0;
+    const int seg = lane / L;
+    const int row = seg < nrows ? A.iorder[b0 + seg] : -1;
+    fast_rows<P, U>(A, L, lane, row);
+}
+
+// ---- persistent lean path --------------------------------------------------------
+// Exactly gridDim.x*WG/64 waves, launched cooperatively so all are resident;
+// wave g owns work items g, g+W, g+2W, ... and processes them in increasing
+// order (MODE 0: rows in natural order, 1: rows in iorder, 2: multirow waves
+// of the analysis).  No dispatch per row and no ticket atomics; W sets how
+// many waves poll at once.  Deadlock freedom: the lowest unfinished item is
+// at the head of its wave's queue and every item it depends on is lower.
+template <class P, int WG, int U, int MODE>
+__global__ __launch_bounds__(WG) void k_persist(SolveArgs A) {
+    const int W = gridDim.x * (WG / kWave);
+    const int wave = blockIdx.x * (WG / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int count = (MODE == 2) ? A.n_waves : A.n;
+    for (int t = wave; t < count; t += W) {
+        if constexpr (MODE == 2) {
+            const int vs = A.wsize[t];
+            const int L = vs == 0 ? 1 : vs;
+            const int b0 = A.wbase[t];
+            const int seg = lane / L;
+            const int row = seg < A.wbase[t + 1] - b0 ? A.iorder[b0 + seg] : -1;
+            fast_rows<P, U>(A, L, lane, row);
+        } else {
+            const int row = (MODE == 1) ? A.iorder[t] : t;
+            fast_rows<P, U>(A, A.L, lane, lane < A.L ? row : -1);
+        }
+    }
+}
+
 // Fill the published words with the unsolved pattern and reset the ticket.
 template <class T>
 __global__ void k_prepare(T* pub, size_t count, T unsolved, int* ticket) {
@@ -392,25 +709,75 @@ __global__ void k_prepare(T* pub, size_t count, T unsolved, int* ticket) {
 }
 
 // ---- host-side launch ----------------------------------------------------------
-enum KernelKind { KK_SIMPLE = 1, KK_ORDER = 2, KK_MULTIROW = 3 };
+enum KernelKind {
+    KK_SIMPLE = 1,
+    KK_ORDER = 2,
+    KK_MULTIROW = 3,
+    KK_FAST_SIMPLE = 4,    // one row per wave, natural order (K == 1)
+    KK_FAST_ORDER = 5,     // one row per wave, level order (K == 1)
+    KK_FAST_MULTIROW = 6,  // multirow waves, no in-wave dependency (K == 1)
+    KK_PERSIST_SIMPLE = 7,    // persistent lean path over rows (K == 1)
+    KK_PERSIST_ORDER = 8,     // persistent lean path over iorder (K == 1)
+    KK_PERSIST_MULTIROW = 9,  // persistent lean path over multirow waves (K == 1)
+};
+
+// Cooperative launch of a persistent kernel with A.persist_grid workgroups,
+// clamped to what can be resident at once.
+template <class P, int WG, int U, int MODE>
+hipError_t launch_persist(const SolveArgs& A, hipStream_t s) {
+    const void* f = reinterpret_cast<const void*>(&k_persist<P, WG, U, MODE>);
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, A.dyn_lds);
+    if (e != hipSuccess) return e;
+    const long long cap = (long long)per_cu * cus;
+    unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(A.persist_grid, cap));
+    SolveArgs a = A;
+    void* args[] = {&a};
+    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(WG), args, A.dyn_lds, s);
+}
 
 template <class P, int K, int WG, int U>
 hipError_t launch_k(int kind, const SolveArgs& A, hipStream_t s) {
     unsigned grid;
-    if (kind == KK_MULTIROW)
+    if (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW)
         grid = ceil_div(A.n_waves, WG / kWave);
+    else if (kind == KK_FAST_SIMPLE || kind == KK_FAST_ORDER)
+        grid = ceil_div(A.n, WG / kWave);
     else
         grid = ceil_div(A.n, WG >> (A.lgL + kLog2<K>()));
     if (grid == 0) return hipSuccess;
+    if constexpr (K == 1) {
+        switch (kind) {
+            case KK_FAST_SIMPLE:
+                hipLaunchKernelGGL((k_fast_row<P, WG, U, false>), dim3(grid), dim3(WG), A.dyn_lds, s, A);
+                return hipGetLastError();
+            case KK_FAST_ORDER:
+                hipLaunchKernelGGL((k_fast_row<P, WG, U, true>), dim3(grid), dim3(WG), A.dyn_lds, s, A);
+                return hipGetLastError();
+            case KK_FAST_MULTIROW:
+                hipLaunchKernelGGL((k_fast_multirow<P, WG, U>), dim3(grid), dim3(WG), A.dyn_lds, s, A);
+                return hipGetLastError();
+            case KK_PERSIST_SIMPLE:
+                return launch_persist<P, WG, U, 0>(A, s);
+            case KK_PERSIST_ORDER:
+                return launch_persist<P, WG, U, 1>(A, s);
+            case KK_PERSIST_MULTIROW:
+                return launch_persist<P, WG, U, 2>(A, s);
+            default:
+                break;
+        }
+    }
     switch (kind) {
         case KK_SIMPLE:
-            hipLaunchKernelGGL((k_simple<P, K, WG, U>), dim3(grid), dim3(WG), 0, s, A);
+            hipLaunchKernelGGL((k_simple<P, K, WG, U>), dim3(grid), dim3(WG), A.dyn_lds, s, A);
             break;
         case KK_ORDER:
-            hipLaunchKernelGGL((k_order<P, K, WG, U>), dim3(grid), dim3(WG), 0, s, A);
+            hipLaunchKernelGGL((k_order<P, K, WG, U>), dim3(grid), dim3(WG), A.dyn_lds, s, A);
             break;
         case KK_MULTIROW:
-            hipLaunchKernelGGL((k_multirow<P, K, WG, U>), dim3(grid), dim3(WG), 0, s, A);
+            hipLaunchKernelGGL((k_multirow<P, K, WG, U>), dim3(grid), dim3(WG), A.dyn_lds, s, A);
             break;
         default:
             return hipErrorInvalidValue;
@@ -426,6 +793,8 @@ hipError_t launch_solve(int kind, int K, int wg, int u, const SolveArgs& A, hipS
     if (K == 1) {
         if (wg == 1024)
             return (u == 4) ? launch_k<P, 1, 1024, 4>(kind, A, s) : launch_k<P, 1, 1024, 1>(kind, A, s);
+        if (wg == 64)
+            return (u == 4) ? launch_k<P, 1, 64, 4>(kind, A, s) : launch_k<P, 1, 64, 1>(kind, A, s);
         return (u == 4) ? launch_k<P, 1, 256, 4>(kind, A, s) : launch_k<P, 1, 256, 1>(kind, A, s);
     }
     if constexpr (MAXK > 1) {

@@ -198,13 +198,15 @@ def csr_L_solve_mixed(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, ref
                                     mat.nr, k, refine_steps, stream), "csr_L_solve_mixed")
 
 
-def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1) -> Config:
-    return Config(kernel, lanes, wg, fused)
+def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0) -> Config:
+    return Config(kernel, lanes, wg, fused, variant, cu_waves)
 
 
 def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
-          kernel: int = KERNEL_AUTO, lanes: int = 0, wg: int = 0, fused: int = 1, stream=None) -> None:
-    cfg = _cfg(kernel, lanes, wg, fused)
+          kernel: int = KERNEL_AUTO, lanes: int = 0, wg: int = 0, fused: int = 1, variant: int = 0,
+          cu_waves: int = 0, stream=None, cfg: Config | None = None) -> None:
+    """cfg, when given, is used as is (e.g. the result of autotune)."""
+    cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves)
     check(L.lib().sptrsv_solve(byref(mat), byref(info) if info is not None else None, precision, byref(cfg),
                                b.ptr, x.ptr, mat.nr, k, refine_steps, stream), "sptrsv_solve")
 
@@ -238,8 +240,9 @@ def default_config(mat, info, precision: int, k: int = 1) -> Config:
 
 
 def bench_solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
-                reps: int = 5, check_ones: bool = False, kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1) -> BenchResult:
-    cfg = _cfg(kernel, lanes, wg, fused)
+                reps: int = 5, check_ones: bool = False, kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1,
+                variant=0) -> BenchResult:
+    cfg = _cfg(kernel, lanes, wg, fused, variant)
     out = BenchResult()
     check(L.lib().sptrsv_bench_solve(byref(mat), byref(info) if info is not None else None, precision,
                                      byref(cfg), b.ptr, x.ptr, mat.nr, k, refine_steps, reps,
@@ -390,6 +393,28 @@ def device_name(device: int = 0) -> str:
 
 def synchronize() -> None:
     check(L.lib().sptrsv_synchronize(), "synchronize")
+
+
+def set_backoff(mat: SpMat, level: int) -> None:
+    check(L.lib().sptrsv_set_backoff(byref(mat), level), "sptrsv_set_backoff")
+
+
+def set_lds_pad(mat: SpMat, nbytes: int) -> None:
+    check(L.lib().sptrsv_set_lds_pad(byref(mat), nbytes), "sptrsv_set_lds_pad")
+
+
+def trace(mat: SpMat, enable: bool, k: int = 1) -> None:
+    check(L.lib().sptrsv_trace(byref(mat), k, 1 if enable else 0), "sptrsv_trace")
+
+
+def trace_read(mat: SpMat, k: int = 1) -> tuple[np.ndarray, float]:
+    """(n*k, 3) wall-clock stamps (start, reduced, stored) and the clock rate in kHz."""
+    cnt = 3 * mat.nr * k
+    out = np.zeros(cnt, np.int64)
+    rate = ctypes.c_double()
+    check(L.lib().sptrsv_trace_read(byref(mat), out.ctypes.data_as(ctypes.POINTER(c_int64)), cnt,
+                                    byref(rate)), "sptrsv_trace_read")
+    return out.reshape(-1, 3), rate.value
 
 
 def kernel_timer(enable: bool) -> None:

@@ -43,12 +43,13 @@ class Dev:
         self.mat = I.allocate_memory(ia, ja, a)
         self.info = I.multirow_analysis_base_GPU(self.mat, I.ALL) if analyse else None
 
-    def solve(self, b, precision=I.PREC_F64, kernel=I.KERNEL_SIMPLE, k=1, lanes=0, wg=0, fused=1, refine=1):
+    def solve(self, b, precision=I.PREC_F64, kernel=I.KERNEL_SIMPLE, k=1, lanes=0, wg=0, fused=1, refine=1,
+              variant=0, cu_waves=0):
         dt = np.float32 if precision == I.PREC_F32 else np.float64
         db = I.DeviceArray.from_numpy(np.asarray(b, dt))
         dx = I.DeviceArray(self.n * k, dt)
         I.solve(self.mat, self.info, precision, db, dx, k=k, refine_steps=refine, kernel=kernel, lanes=lanes,
-                wg=wg, fused=fused)
+                wg=wg, fused=fused, variant=variant, cu_waves=cu_waves)
         assert I.check_watchdog(self.mat) == 0, L.last_error()
         x = dx.numpy()
         db.free()
@@ -169,8 +170,56 @@ def test_fp64_all_kernels_golden(device, golden, case):
     for name, kern in KERNELS.items():
         for lanes in ([0] if kern == I.KERNEL_MULTIROW else [1, 2, 8, 64]):
             for wg in ([256, 1024] if kern == I.KERNEL_SIMPLE else [256]):
-                x = d.solve(b, kernel=kern, lanes=lanes, wg=wg)
-                assert relerr(x, ref) <= 1e-12, (name, lanes, wg)
+                for variant in (0, 1):
+                    ln = lanes if lanes or variant == 0 else 4
+                    x = d.solve(b, kernel=kern, lanes=ln, wg=wg, variant=variant)
+                    assert relerr(x, ref) <= 1e-12, (name, lanes, wg, variant)
+                    xm = d.solve(b, precision=I.PREC_MIXED, kernel=kern, lanes=ln, wg=wg, variant=variant)
+                    assert relerr(xm, ref) <= 1e-10, (name, lanes, wg, variant)
+    d.close()
+
+
+def test_residency_caps_and_small_workgroups(device):
+    s = I.generate(2, 100000)
+    ref = O.fwd_f64(s.ia, s.ja, s.a, s.b)
+    d = Dev(s.ia, s.ja, s.a)
+    for name, kern in KERNELS.items():
+        for wg in (64, 256):
+            for cw in (1, 2, 8):
+                for variant in (0, 1, 2):
+                    x = d.solve(s.b, kernel=kern, lanes=16, wg=wg, variant=variant, cu_waves=cw)
+                    assert relerr(x, ref) <= 1e-12, (name, wg, cw, variant)
+                    xm = d.solve(s.b, precision=I.PREC_MIXED, kernel=kern, lanes=16, wg=wg, variant=variant,
+                                 cu_waves=cw)
+                    assert relerr(xm, ref) <= 1e-10, (name, wg, cw, variant)
+    d.close()
+
+
+def test_persistent_edge_cases(device):
+    # more waves than rows, a single row, long chains, long rows
+    rng = np.random.default_rng(2)
+    cases = {"one_row": [[]], "chain": [[i - 1] if i else [] for i in range(3000)],
+             "long_row": [[]] * 2000 + [list(range(2000))], "diag": [[]] * 100}
+    for name, rows in cases.items():
+        ia, ja, a = lower_from_rows(rows, rng)
+        b = rng.uniform(-1, 1, len(rows))
+        ref = O.fwd_f64(ia, ja, a, b)
+        d = Dev(ia, ja, a)
+        for kern in KERNELS.values():
+            for cw in (1, 16):
+                assert relerr(d.solve(b, kernel=kern, lanes=8, wg=64, variant=2, cu_waves=cw), ref) <= 1e-12, name
+        d.close()
+
+
+def test_fast_and_general_paths_agree_bitwise(device):
+    # same per-lane folding order and the same reduction tree for equal lanes
+    s = I.generate(3, 60000)
+    d = Dev(s.ia, s.ja, s.a)
+    for lanes in (4, 16, 64):
+        x0 = d.solve(s.b, kernel=I.KERNEL_ORDER, lanes=lanes, variant=0)
+        x1 = d.solve(s.b, kernel=I.KERNEL_ORDER, lanes=lanes, variant=1)
+        x2 = d.solve(s.b, kernel=I.KERNEL_SIMPLE, lanes=lanes, variant=1)
+        assert np.array_equal(x0, x1) and np.array_equal(x1, x2), lanes
     d.close()
 
 
@@ -180,12 +229,16 @@ def test_fp64_fp32_mixed_configs(device, cfg, param):
     ref = O.fwd_f64(s.ia, s.ja, s.a, s.b)
     d = Dev(s.ia, s.ja, s.a)
     for name, kern in KERNELS.items():
-        assert relerr(d.solve(s.b, kernel=kern), ref) <= 1e-12, name
-        x32 = d.solve(s.b.astype(np.float32), precision=I.PREC_F32, kernel=kern)
-        assert relerr(x32.astype(np.float64), ref) <= 1e-5, name
-        for fused in (1, 0):
-            xm = d.solve(s.b, precision=I.PREC_MIXED, kernel=kern, fused=fused)
-            assert relerr(xm, ref) <= 1e-10, (name, fused)
+        for variant in (0, 1):
+            lanes = 0 if variant == 0 else 16
+            assert relerr(d.solve(s.b, kernel=kern, lanes=lanes, variant=variant), ref) <= 1e-12, name
+            x32 = d.solve(s.b.astype(np.float32), precision=I.PREC_F32, kernel=kern, lanes=lanes,
+                          variant=variant)
+            assert relerr(x32.astype(np.float64), ref) <= 1e-5, name
+            for fused in (1, 0):
+                xm = d.solve(s.b, precision=I.PREC_MIXED, kernel=kern, fused=fused, lanes=lanes,
+                             variant=variant)
+                assert relerr(xm, ref) <= 1e-10, (name, fused, variant)
     d.close()
 
 
