@@ -57,13 +57,40 @@ def bytes_method_convention(n, nnz, k):
     return 2 * f32 + res + upd
 
 
-KERNEL_NAMES = {1: "simple", 2: "order", 3: "multirow"}
+def bytes_jacobi_sweeps(n, nnz, k, sweeps, outer):
+    """Average algorithmic bytes of one Jacobi sweep launch over a solve.
+
+    A sweep reads ia, ja, a32 (strict lower part and diagonal), the previous
+    iterate d (4kn, each entry once) and r32 (4kn).  Inner sweeps write d
+    (4kn); the last sweep of a step writes x in FP64 (8kn) and, after the
+    first step, also reads it (8kn)."""
+    base = 4 * (n + 1) + 8 * nnz + 8 * k * n
+    inner = (sweeps - 2) * outer * (base + 4 * k * n)
+    last = (base + 8 * k * n) + (outer - 1) * (base + 16 * k * n)
+    return (inner + last) / ((sweeps - 1) * outer)
+
+
+def bytes_jacobi_method(n, nnz, k, sweeps, outer):
+    """Whole Jacobi-refinement solve: per step one FP64 residual (the first
+    reads no x) and sweeps-1 sweeps, then the certifying FP64 residual."""
+    res = 4 * (n + 1) + 12 * nnz + 16 * k * n + 4 * k * n + 4 * k * n   # b, x, r32, d0
+    first_res = res - 8 * k * n
+    sweeps_b = bytes_jacobi_sweeps(n, nnz, k, sweeps, outer) * (sweeps - 1) * outer
+    cert = 4 * (n + 1) + 12 * nnz + 16 * k * n + 4 * k * n
+    return first_res + (outer - 1) * res + sweeps_b + cert
+
+
+KERNEL_NAMES = {1: "simple", 2: "order", 3: "multirow", 4: "jacobi"}
 
 
 def cfg_dict(c) -> dict:
-    return {"kernel": KERNEL_NAMES.get(c.kernel, c.kernel), "lanes": c.lanes, "wg": c.wg,
-            "fused": c.fused, "variant": "one-row-per-wave" if c.variant else "multi-row-wave",
-            "cu_waves": c.cu_waves}
+    d = {"kernel": KERNEL_NAMES.get(c.kernel, c.kernel), "lanes": c.lanes, "wg": c.wg}
+    if c.kernel == 4:
+        d.update(sweeps=c.sweeps, outer=c.outer)
+    else:
+        d.update(fused=c.fused, variant="one-row-per-wave" if c.variant else "multi-row-wave",
+                 cu_waves=c.cu_waves)
+    return d
 
 
 def cpu_baseline(s, budget_s: float):
@@ -198,8 +225,15 @@ def main() -> int:
     n, nnz = s.n, s.nnz
     gnnz_m = k * nnz / (msm * 1e-3) / 1e9
     gnnz_64 = k * nnz / (ms64 * 1e-3) / 1e9
-    b_fused = bytes_mixed_fused(n, nnz, k)
-    achieved = b_fused / (kmsm * 1e-3) / 1e9
+    cm = tuned[I.PREC_MIXED]
+    if cm.kernel == I.KERNEL_JACOBI:
+        b_launch = bytes_jacobi_sweeps(n, nnz, k, cm.sweeps, cm.outer)
+        b_method = bytes_jacobi_method(n, nnz, k, cm.sweeps, cm.outer)
+        kname = "Jacobi sweep (FP32, k_jac_sweep) inside FP64 refinement"
+    else:
+        b_launch = b_method = bytes_mixed_fused(n, nnz, k)
+        kname = "sync-free fused mixed-precision solve"
+    achieved = b_launch / (kmsm * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline(s, args.cpu_seconds)
     out = {
         "metric": METRIC,
@@ -221,12 +255,14 @@ def main() -> int:
             "rel_err_mixed": errm, "rel_err_f64": err64,
             "plain_f64_gnnz_s": round(gnnz_64, 4), "plain_f64_ms": round(ms64, 4),
             "speedup_mixed_vs_f64": round(ms64 / msm, 3),
-            "hbm_gbs_mixed": round(b_fused / (msm * 1e-3) / 1e9, 2),
+            "hbm_gbs_mixed": round(b_method / (msm * 1e-3) / 1e9, 2),
+            "bytes_mixed_method": int(b_method),
             "hbm_gbs_f64": round(bytes_f64(n, nnz, k) / (ms64 * 1e-3) / 1e9, 2),
             "bytes_method_separate_pass_convention": bytes_method_convention(n, nnz, k),
             "cold_cache_ms": {"mixed": round(coldm, 4), "f64": round(cold64, 4)},
             "t_dep_ns": round(tdep, 1),
             "latency_bound_fraction_mixed": round(nlevs * tdep * 1e-6 / msm, 3),
+            "latency_bound_fraction_f64": round(nlevs * tdep * 1e-6 / ms64, 3),
             "power_w": power,
             "rocsparse_csrsv": rocsparse,
         },
@@ -237,9 +273,9 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": None,
-            "kernel": "sync-free fused mixed-precision solve",
+            "kernel": kname,
             "kernel_ms": round(kmsm, 4),
-            "bytes_per_launch": b_fused,
+            "bytes_per_launch": int(b_launch),
         },
         "cpu_baseline": cpu,
     }

@@ -39,6 +39,7 @@ typedef struct ihipStream_t* hipStream_t;
 #define SPTRSV_E_UNAVAILABLE (-8)        /* optional runtime (amdsmi, rocSPARSE) absent */
 #define SPTRSV_E_IO (-9)                 /* Matrix Market read/parse failure */
 #define SPTRSV_E_CAPACITY (-10)          /* request larger than a fixed limit */
+#define SPTRSV_E_NOT_CONVERGED (-11)     /* iterative solve missed its certified bound */
 
 const char* sptrsv_status_string(int status);
 /* Detail of the last error raised on the calling thread ("" if none). */
@@ -159,6 +160,14 @@ int csr_L_solve_multirow_f32(sp_mat_t* mat, dfr_analysis_info_t* info, const flo
 #define SPTRSV_KERNEL_SIMPLE 1
 #define SPTRSV_KERNEL_ORDER 2
 #define SPTRSV_KERNEL_MULTIROW 3
+/* MIXED only: the FP32 bulk solve of each refinement step is `sweeps`
+ * Jacobi sweeps (d <- D^-1 (r - N d), N the strict lower part) instead of a
+ * sync-free substitution, `outer` refinement steps (FP64 residual, FP32
+ * sweeps, FP64 update), then an FP64 residual certifies the result on the
+ * device: |x - x*|_inf <= |r|_inf / delta with delta = min_i (|L_ii| -
+ * sum_j |L_ij|) > 0 (strict row diagonal dominance).  A miss is reported by
+ * sptrsv_check as SPTRSV_E_NOT_CONVERGED. */
+#define SPTRSV_KERNEL_JACOBI 4
 
 #define SPTRSV_PREC_F64 0    /* plain FP64 sync-free solve */
 #define SPTRSV_PREC_F32 1    /* plain FP32 sync-free solve */
@@ -183,7 +192,15 @@ typedef struct {
     int fused;
     int variant;
     int cu_waves;
+    int sweeps;   /* JACOBI: FP32 sweeps per refinement step */
+    int outer;    /* JACOBI: refinement steps */
 } sptrsv_config_t;
+
+/* Relative accuracy the JACOBI path certifies (default 1e-10). */
+int sptrsv_set_tolerance(sp_mat_t* mat, double rel_tol);
+/* Row diagonal-dominance margin delta of the matrix (computed on first use;
+ * <= 0 means the JACOBI path cannot certify and is not offered). */
+int sptrsv_dominance_margin(sp_mat_t* mat, double* delta);
 
 /* Generic solve with k right-hand sides (row-major n x k, x[i*k + r]).
  * precision selects the arithmetic; for SPTRSV_PREC_F32, b and x are float*,

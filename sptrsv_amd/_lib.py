@@ -28,7 +28,8 @@ E_CAPACITY = -10
 # MODE (dfr_syncfree.h:7-12)
 SIMPLE, ORDER, MULTIROW, ALL = 0, 1, 2, 3
 
-KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_ORDER, KERNEL_MULTIROW = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_ORDER, KERNEL_MULTIROW, KERNEL_JACOBI = 0, 1, 2, 3, 4
+E_NOT_CONVERGED = -11
 PREC_F64, PREC_F32, PREC_MIXED = 0, 1, 2
 
 
@@ -58,7 +59,7 @@ class AnalysisInfo(Structure):  # dfr_analysis_info_t, dfr_syncfree.h:25-41
 
 class Config(Structure):  # sptrsv_config_t
     _fields_ = [("kernel", c_int), ("lanes", c_int), ("wg", c_int), ("fused", c_int), ("variant", c_int),
-                ("cu_waves", c_int)]
+                ("cu_waves", c_int), ("sweeps", c_int), ("outer", c_int)]
 
 
 class BenchResult(Structure):  # sptrsv_bench_t
@@ -139,6 +140,8 @@ _PROTOS = {
     "sptrsv_measure_dep_latency": (c_int, [c_int, POINTER(c_float)]),
     "sptrsv_set_backoff": (c_int, [POINTER(SpMat), c_int]),
     "sptrsv_set_lds_pad": (c_int, [POINTER(SpMat), c_int]),
+    "sptrsv_set_tolerance": (c_int, [POINTER(SpMat), c_double]),
+    "sptrsv_dominance_margin": (c_int, [POINTER(SpMat), POINTER(c_double)]),
     "sptrsv_trace": (c_int, [POINTER(SpMat), c_int, c_int]),
     "sptrsv_trace_read": (c_int, [POINTER(SpMat), POINTER(c_int64), c_size_t, POINTER(c_double)]),
 }

@@ -71,6 +71,11 @@ static unsigned stream_grid(size_t count, unsigned threads = 256) {
     return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 8192));
 }
 
+int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, double* x, hipStream_t s);
+int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best_ms);
+int jacobi_margin(sp_mat_t* m, double* delta);
+int reserve_jacobi(sp_mat_t* m, int k);
+
 // ---- configuration --------------------------------------------------------------
 
 static int ilog2(int v) {
@@ -153,7 +158,8 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     } else if (pv->tuned[precision] && pv->tuned_k[precision] == k &&
                (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_SIMPLE ||
                 (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_ORDER && has_order(info)) ||
-                (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_MULTIROW && has_waves(info)))) {
+                (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_MULTIROW && has_waves(info)) ||
+                (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_JACOBI && precision == SPTRSV_PREC_MIXED))) {
         *out = pv->tuned_cfg[precision];
         if (cfg) out->fused = cfg->fused;
     } else {
@@ -165,6 +171,13 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
         sptrsv_config_t d;
         default_config(m, info, precision, k, &d);
         out->lanes = d.lanes;
+    }
+    if (out->kernel == SPTRSV_KERNEL_JACOBI && (out->sweeps <= 0 || out->outer <= 0)) {
+        // schedule not given: the tuned one, else 12 sweeps x 4 refinement steps
+        const sptrsv_config_t& t = pv->tuned_cfg[precision];
+        const bool tj = pv->tuned[precision] && pv->tuned_k[precision] == k && t.kernel == SPTRSV_KERNEL_JACOBI;
+        if (out->sweeps <= 0) out->sweeps = tj ? t.sweeps : 12;
+        if (out->outer <= 0) out->outer = tj ? t.outer : 4;
     }
     if (out->wg != 64 && out->wg != 256 && out->wg != 1024) out->wg = 256;
     if (K > 1) {
@@ -178,8 +191,10 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
         return fail(SPTRSV_E_NOT_ANALYSED, "ORDER solve needs an analysis with iorder");
     if (out->kernel == SPTRSV_KERNEL_MULTIROW && !has_waves(info))
         return fail(SPTRSV_E_NOT_ANALYSED, "MULTIROW solve needs a MULTIROW/ALL analysis");
-    if (out->kernel < SPTRSV_KERNEL_SIMPLE || out->kernel > SPTRSV_KERNEL_MULTIROW)
+    if (out->kernel < SPTRSV_KERNEL_SIMPLE || out->kernel > SPTRSV_KERNEL_JACOBI)
         return fail(SPTRSV_E_INVALID_ARG, "unknown kernel");
+    if (out->kernel == SPTRSV_KERNEL_JACOBI && precision != SPTRSV_PREC_MIXED)
+        return fail(SPTRSV_E_INVALID_ARG, "JACOBI is the bulk solver of the MIXED path only");
     return SPTRSV_OK;
 }
 
@@ -348,6 +363,7 @@ static int solve_impl(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     // mixed precision
     const double* bd = static_cast<const double*>(b);
     double* xd = static_cast<double*>(x);
+    if (c.kernel == SPTRSV_KERNEL_JACOBI) return jacobi_solve(m, c, k, bd, xd, s);
     int extra = refine_steps;
     if (c.fused) {
         SPT_TRY(reserve(m, k, true, refine_steps > 1));
@@ -458,21 +474,28 @@ int csr_L_solve_mixed(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b,
 
 int sptrsv_reserve(sp_mat_t* mat, int k) {
     if (!mat || !mat->priv || k < 1 || k > kWave) return fail(SPTRSV_E_INVALID_ARG, "bad reserve");
-    return reserve(mat, k, true, true);
+    SPT_TRY(reserve(mat, k, true, true));
+    if (mat->a) SPT_TRY(reserve_jacobi(mat, k));
+    return SPTRSV_OK;
 }
 
 int sptrsv_check(sp_mat_t* mat, hipStream_t stream) {
     if (!mat || !mat->priv) return fail(SPTRSV_E_INVALID_ARG, "matrix not allocated");
     MatPriv* pv = priv_of(mat);
     SPT_HIP(hipStreamSynchronize(stream));
-    int err = 0;
-    SPT_HIP(hipMemcpy(&err, pv->counters + 1, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) {
+    int w[4] = {0, 0, 0, 0};
+    SPT_HIP(hipMemcpy(w, pv->counters, sizeof(w), hipMemcpyDeviceToHost));
+    if (w[1] || w[3]) {
         int zero = 0;
-        SPT_HIP(hipMemcpy(pv->counters + 1, &zero, sizeof(int), hipMemcpyHostToDevice));
+        if (w[1]) SPT_HIP(hipMemcpy(pv->counters + 1, &zero, sizeof(int), hipMemcpyHostToDevice));
+        if (w[3]) SPT_HIP(hipMemcpy(pv->counters + 3, &zero, sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (w[1])
         return fail(SPTRSV_E_WATCHDOG, "a dependency wait exceeded the watchdog bound (input "
                                        "malformed or bound too small); affected rows hold NaN");
-    }
+    if (w[3])
+        return fail(SPTRSV_E_NOT_CONVERGED, "JACOBI refinement did not reach the certified tolerance; "
+                                            "re-solve with a sync-free kernel or more refinement steps");
     return SPTRSV_OK;
 }
 
@@ -646,12 +669,26 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
             }
         }
     }
-    hipEventDestroy(e0);
+    (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(b);
     (void)hipFree(x);
     if (status != SPTRSV_OK) return status;
     SPT_TRY(sptrsv_check(mat, nullptr));
+    // Mixed precision on a strictly diagonally dominant matrix: also time the
+    // certified Jacobi refinement and keep it when it is faster.
+    double delta = 0;
+    if (precision == SPTRSV_PREC_MIXED && mat->a && jacobi_margin(mat, &delta) == SPTRSV_OK && delta > 0) {
+        sptrsv_config_t jc{};
+        float jms = 0;
+        int st = jacobi_tune(mat, k, reps, &jc, &jms);
+        if (st == SPTRSV_OK && jms < best_t) {
+            best_t = jms;
+            bc = jc;
+        } else if (st != SPTRSV_OK && st != SPTRSV_E_NOT_CONVERGED) {
+            return st;
+        }
+    }
     pv->tuned[precision] = true;
     pv->tuned_cfg[precision] = bc;
     pv->tuned_k[precision] = k;

@@ -54,6 +54,15 @@ struct MatPriv {
     size_t trace_len = 0;
     int backoff = 0;
     unsigned lds_pad = 0;        // extra LDS per workgroup, caps resident waves
+    // JACOBI refinement
+    double tol = 1e-10;          // certified relative accuracy
+    double margin = 0.0;         // min_i |L_ii| - sum_j |L_ij|
+    bool margin_known = false;
+    float* jac_r = nullptr;      // FP32 residual, n*k
+    float* jac_a = nullptr;      // Jacobi iterates (ping-pong), n*k each
+    float* jac_b = nullptr;
+    size_t jac_len = 0;
+    unsigned long long* jac_norms = nullptr;  // max |r|, max |x| (double bits)
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};
     int tuned_k[3] = {0, 0, 0};

@@ -78,6 +78,9 @@ static void free_mat(sp_mat_t* m) {
         if (pv->corr) (void)hipFree(pv->corr);
         if (pv->resid) (void)hipFree(pv->resid);
         if (pv->trace) (void)hipFree(pv->trace);
+        for (float* p : {pv->jac_r, pv->jac_a, pv->jac_b})
+            if (p) (void)hipFree(p);
+        if (pv->jac_norms) (void)hipFree(pv->jac_norms);
         delete pv;
     }
     memset(m, 0, sizeof(*m));
@@ -279,6 +282,7 @@ const char* sptrsv_status_string(int status) {
         case SPTRSV_E_UNAVAILABLE: return "optional runtime unavailable";
         case SPTRSV_E_IO: return "Matrix Market read error";
         case SPTRSV_E_CAPACITY: return "request exceeds a fixed capacity";
+        case SPTRSV_E_NOT_CONVERGED: return "iterative solve missed its certified accuracy";
         default: return "unknown status";
     }
 }

@@ -15,13 +15,13 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib as L
-from ._lib import (ALL, KERNEL_AUTO, KERNEL_MULTIROW, KERNEL_ORDER, KERNEL_SIMPLE, MULTIROW, ORDER,
-                   PREC_F32, PREC_F64, PREC_MIXED, SIMPLE, AnalysisInfo, BenchResult, Config, SpMat,
+from ._lib import (ALL, E_NOT_CONVERGED, KERNEL_AUTO, KERNEL_JACOBI, KERNEL_MULTIROW, KERNEL_ORDER,
+                   KERNEL_SIMPLE, MULTIROW, ORDER, PREC_F32, PREC_F64, PREC_MIXED, SIMPLE, AnalysisInfo, BenchResult, Config, SpMat,
                    SptrsvError, check)
 
 __all__ = [
     "SIMPLE", "ORDER", "MULTIROW", "ALL", "KERNEL_AUTO", "KERNEL_SIMPLE", "KERNEL_ORDER",
-    "KERNEL_MULTIROW", "PREC_F64", "PREC_F32", "PREC_MIXED", "SpMat", "AnalysisInfo", "Config",
+    "KERNEL_MULTIROW", "KERNEL_JACOBI", "E_NOT_CONVERGED", "PREC_F64", "PREC_F32", "PREC_MIXED", "SpMat", "AnalysisInfo", "Config",
     "SptrsvError", "DeviceArray", "allocate_memory", "allocate_memory_f32", "run_analysis",
     "run_solver", "free_device_analysis", "multirow_analysis_base_GPU", "csr_L_solve_simple",
     "csr_L_solve_order", "csr_L_solve_multirow", "csr_L_solve_simple_f32", "csr_L_solve_order_f32",
@@ -29,7 +29,7 @@ __all__ = [
     "default_config", "set_watchdog_ms", "check_watchdog", "reserve", "generate",
     "generate_batch_member", "read_mtx_lower", "Batch", "RocsparseBaseline", "power_start",
     "power_end", "bench_solve", "device_name", "synchronize", "flush_cache", "Timer",
-    "kernel_timer", "kernel_timer_read", "dep_latency_ns",
+    "kernel_timer", "kernel_timer_read", "dep_latency_ns", "set_tolerance", "dominance_margin",
 ]
 
 
@@ -198,15 +198,15 @@ def csr_L_solve_mixed(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, ref
                                     mat.nr, k, refine_steps, stream), "csr_L_solve_mixed")
 
 
-def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0) -> Config:
-    return Config(kernel, lanes, wg, fused, variant, cu_waves)
+def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0, sweeps=0, outer=0) -> Config:
+    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer)
 
 
 def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
           kernel: int = KERNEL_AUTO, lanes: int = 0, wg: int = 0, fused: int = 1, variant: int = 0,
-          cu_waves: int = 0, stream=None, cfg: Config | None = None) -> None:
+          cu_waves: int = 0, sweeps: int = 0, outer: int = 0, stream=None, cfg: Config | None = None) -> None:
     """cfg, when given, is used as is (e.g. the result of autotune)."""
-    cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves)
+    cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer)
     check(L.lib().sptrsv_solve(byref(mat), byref(info) if info is not None else None, precision, byref(cfg),
                                b.ptr, x.ptr, mat.nr, k, refine_steps, stream), "sptrsv_solve")
 
@@ -220,7 +220,7 @@ def set_watchdog_ms(mat: SpMat, ms: float) -> None:
 
 
 def check_watchdog(mat: SpMat) -> int:
-    """Returns the status (0, or E_WATCHDOG) without raising."""
+    """Returns the status of the last solves (0, E_WATCHDOG or E_NOT_CONVERGED) without raising."""
     return L.lib().sptrsv_check(byref(mat), None)
 
 
@@ -397,6 +397,16 @@ def synchronize() -> None:
 
 def set_backoff(mat: SpMat, level: int) -> None:
     check(L.lib().sptrsv_set_backoff(byref(mat), level), "sptrsv_set_backoff")
+
+
+def set_tolerance(mat: SpMat, rel_tol: float) -> None:
+    check(L.lib().sptrsv_set_tolerance(byref(mat), rel_tol), "sptrsv_set_tolerance")
+
+
+def dominance_margin(mat: SpMat) -> float:
+    d = ctypes.c_double()
+    check(L.lib().sptrsv_dominance_margin(byref(mat), byref(d)), "sptrsv_dominance_margin")
+    return d.value
 
 
 def set_lds_pad(mat: SpMat, nbytes: int) -> None:

@@ -1,0 +1,139 @@
+"""GPU tests of the certified Jacobi-refinement path (KERNEL_JACOBI, mixed
+precision): FP32 Jacobi sweeps inside FP64 iterative refinement, with the
+a-posteriori bound |x - x*|_inf <= |b - L x|_inf / delta checked on the
+device.
+
+The result is compared with the oracle's serial FP64 forward substitution
+(the reference's solution) at the mixed-precision bar, 1e-10 relative
+max-norm error.  The certificate itself is pinned by soundness: whenever it
+reports success, the true error against the oracle is within the tolerance.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from sptrsv_amd import _lib as L
+from sptrsv_amd import interface as I
+from test_gpu_solve import Dev, relerr
+
+pytestmark = pytest.mark.gpu
+
+J = dict(precision=I.PREC_MIXED, kernel=I.KERNEL_JACOBI)
+
+
+def margin_np(ia, a):
+    n = len(ia) - 1
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    diag = a[ia[1:] - 1]
+    off = np.bincount(rows, weights=np.abs(a), minlength=n) - np.abs(diag)
+    return float(np.min(np.abs(diag) - off))
+
+
+@pytest.mark.parametrize("cfg,param", [(1, 64), (2, 200000), (3, 200000), (4, 24)])
+def test_jacobi_certified_configs(cfg, param):
+    s = I.generate(cfg, param)
+    ref = O.fwd_f64(s.ia, s.ja, s.a, s.b, k=s.k)
+    d = Dev(s.ia, s.ja, s.a)
+    assert I.dominance_margin(d.mat) == pytest.approx(margin_np(s.ia, s.a), rel=1e-12)
+    for lanes in (4, 16):
+        x = d.solve(s.b, k=s.k, lanes=lanes, sweeps=16, outer=4, **J)
+        assert relerr(x, ref) <= 1e-10, (cfg, lanes)
+    d.close()
+
+
+def test_jacobi_known_answer():
+    """x = 1 on the reference's known-answer input, to the refinement's
+    tolerance: exact equality is a property of direct substitution, which
+    the sync-free kernels keep; an iterative solve stops within rounding."""
+    s = I.generate(1)
+    d = Dev(s.ia, s.ja, s.a)
+    x = d.solve(s.b, lanes=4, sweeps=12, outer=3, **J)
+    assert np.max(np.abs(x - 1.0)) <= 1e-12
+    d.close()
+
+
+def test_jacobi_certificate_is_sound():
+    """For every schedule the device accepts, the true error is within tol."""
+    s = I.generate(3, 100000)
+    ref = O.fwd_f64(s.ia, s.ja, s.a, s.b)
+    d = Dev(s.ia, s.ja, s.a)
+    accepted = rejected = 0
+    for sweeps in (2, 4, 8):
+        for outer in (1, 2, 3, 4, 6):
+            db = I.DeviceArray.from_numpy(s.b)
+            dx = I.DeviceArray(s.n, np.float64)
+            I.solve(d.mat, None, I.PREC_MIXED, db, dx, kernel=I.KERNEL_JACOBI, lanes=8, sweeps=sweeps,
+                    outer=outer)
+            st = I.check_watchdog(d.mat)
+            err = relerr(dx.numpy(), ref)
+            db.free()
+            dx.free()
+            assert st in (0, I.E_NOT_CONVERGED), L.last_error()
+            if st == 0:
+                accepted += 1
+                assert err <= 1e-10, (sweeps, outer, err)
+            else:
+                rejected += 1
+    assert accepted > 0 and rejected > 0        # both verdicts are exercised
+    assert I.check_watchdog(d.mat) == 0         # the flag is cleared once read
+    d.close()
+
+
+def test_jacobi_tolerance_setting():
+    s = I.generate(2, 50000)
+    ref = O.fwd_f64(s.ia, s.ja, s.a, s.b)
+    d = Dev(s.ia, s.ja, s.a)
+    I.set_tolerance(d.mat, 1e-4)
+    x = d.solve(s.b, lanes=8, sweeps=12, outer=1, **J)  # accepted at the loose tolerance
+    assert relerr(x, ref) <= 1e-4
+    I.set_tolerance(d.mat, 1e-14)
+    d.solve(s.b, lanes=8, sweeps=12, outer=1, expect=I.E_NOT_CONVERGED, **J)
+    with pytest.raises(L.SptrsvError):
+        I.set_tolerance(d.mat, 0.0)
+    d.close()
+
+
+@pytest.mark.parametrize("k", [1, 3, 16, 64])
+def test_jacobi_multi_rhs(k):
+    s = I.generate(4, 20)
+    rng = np.random.default_rng(k)
+    B = rng.uniform(-1, 1, s.n * k)
+    ref = O.fwd_f64(s.ia, s.ja, s.a, B, k=k)
+    d = Dev(s.ia, s.ja, s.a, analyse=False)
+    x = d.solve(B, k=k, lanes=4, sweeps=12, outer=4, **J)
+    assert relerr(x, ref) <= 1e-10, k
+    d.close()
+
+
+def test_jacobi_rejects_non_dominant():
+    ia = np.array([0, 1, 3, 6], np.int32)
+    ja = np.array([0, 0, 1, 0, 1, 2], np.int32)
+    a = np.array([1.0, -0.75, 1.0, -0.75, -0.75, 1.0])      # row 2: 1 - 1.5 < 0
+    d = Dev(ia, ja, a, analyse=False)
+    assert I.dominance_margin(d.mat) == pytest.approx(-0.5)
+    db = I.DeviceArray.from_numpy(np.ones(3))
+    dx = I.DeviceArray(3, np.float64)
+    with pytest.raises(L.SptrsvError) as e:
+        I.solve(d.mat, None, I.PREC_MIXED, db, dx, kernel=I.KERNEL_JACOBI, lanes=2, sweeps=4, outer=2)
+    assert e.value.status == L.E_INVALID_ARG
+    # the sync-free kernels still solve it
+    I.solve(d.mat, d.info, I.PREC_MIXED, db, dx, kernel=I.KERNEL_SIMPLE)
+    ref = O.fwd_f64(ia, ja, a, np.ones(3))
+    assert relerr(dx.numpy(), ref) <= 1e-12
+    db.free()
+    dx.free()
+
+
+def test_jacobi_autotune_and_batch_members():
+    for m in range(4):
+        s = I.generate_batch_member(m, 2000, 20000, seed=7)
+        ref = O.fwd_f64(s.ia, s.ja, s.a, s.b)
+        d = Dev(s.ia, s.ja, s.a)
+        cfg, ms = I.autotune(d.mat, d.info, I.PREC_MIXED, reps=2)
+        x = d.solve(s.b, precision=I.PREC_MIXED, kernel=I.KERNEL_AUTO)
+        assert relerr(x, ref) <= 1e-10, (m, cfg.kernel)
+        if cfg.kernel == I.KERNEL_JACOBI:
+            assert cfg.sweeps >= 2 and cfg.outer >= 1
+        d.close()
