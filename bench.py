@@ -32,6 +32,8 @@ WORKLOADS = {
     2: ("random lower CSR n=1e6, Poisson(8) off-diagonals in [i-5000,i), U(-1,1), diag 1+sum|off|, 1 RHS", 1000000),
     3: ("power-law lower CSR n=2e6, Zipf(1.8) off-diagonals (cap 2000) in [0,i), diag 1+sum|off|, 1 RHS", 2000000),
     4: ("3D 7-point Laplacian lower factor 128^3 (n=2097152), unit diag, off -1/6, 16 RHS", 128),
+    5: ("batch of 256 independent lower CSR systems, n~U[2000,200000]; even members banded (bandwidth U[1,32]), "
+        "odd members Poisson(6) off-diagonals in [0,i); U(-1,1), diag 1+sum|off|, 1 RHS each", 256),
 }
 
 
@@ -110,12 +112,109 @@ def cpu_baseline(s, budget_s: float):
                       f"{os.cpu_count()} cores, 1 used"}
 
 
+def run_batch(args, I, O) -> int:
+    """Workload 5: 256 independent systems in one captured graph (MIXED, per-matrix
+    kernel choice) against the same systems solved one after another in plain FP64."""
+    desc, nmat = WORKLOADS[5]
+    t0 = time.time()
+    systems = [I.generate_batch_member(m, 2000, 200000, seed=5) for m in range(nmat)]
+    rows = sum(x.n for x in systems)
+    nnz = sum(x.nnz for x in systems)
+    log(f"[bench] generated {nmat} systems: rows={rows} nnz={nnz} ({time.time() - t0:.1f}s)")
+    bt = I.Batch(systems)
+    B = np.concatenate([x.b for x in systems])
+    db = I.DeviceArray.from_numpy(B)
+    dx = I.DeviceArray(rows, np.float64)
+    t0 = time.time()
+    bt.tune(I.PREC_MIXED, tune=not args.no_tune)
+    kinds = [bt.choice(m).kernel for m in range(nmat)]
+    jc = next((bt.choice(m) for m in range(nmat) if kinds[m] == I.KERNEL_JACOBI), None)
+    log(f"[bench] tuned in {time.time() - t0:.1f}s: {kinds.count(I.KERNEL_JACOBI)} Jacobi members"
+        + (f" (sweeps {jc.sweeps}, steps {jc.outer})" if jc else ""))
+    for _ in range(args.warmup):
+        bt.solve(I.PREC_MIXED, db, dx, graph=True)
+    I.synchronize()
+    I.kernel_timer(True)
+    timer = I.Timer()
+    timer.start()
+    for _ in range(args.steps):
+        bt.solve(I.PREC_MIXED, db, dx, graph=True)
+    msm = timer.stop() / args.steps
+    kms, launches = I.kernel_timer_read()
+    I.kernel_timer(False)
+    I.synchronize()
+    assert bt.check() == 0
+    x = dx.numpy()
+    errs = []
+    for m, sm in enumerate(systems):
+        ref = O.fwd_f64(sm.ia, sm.ja, sm.a, sm.b)
+        xm = x[bt.row_off[m]:bt.row_off[m + 1]]
+        errs.append(float(np.max(np.abs(xm - ref)) / np.max(np.abs(ref))))
+    log(f"[bench] batched MIXED graph: {msm:.3f} ms, max member rel err {max(errs):.2e}")
+    # every member at once, plain FP64 sync-free (heuristic lanes), one launch
+    bt.tune(I.PREC_F64, tune=False)
+    bt.solve(I.PREC_F64, db, dx, graph=False)
+    I.synchronize()
+    timer = I.Timer()
+    timer.start()
+    for _ in range(max(1, args.steps // 4)):
+        bt.solve(I.PREC_F64, db, dx, graph=False)
+    ms64b = timer.stop() / max(1, args.steps // 4)
+    assert bt.check() == 0
+    # sequential plain FP64: one system at a time, each its own matrix and launch
+    mats = [I.allocate_memory(sm.ia, sm.ja, sm.a) for sm in systems]
+    bufs = [(I.DeviceArray.from_numpy(sm.b), I.DeviceArray(sm.n, np.float64)) for sm in systems]
+    I.solve(mats[0], None, I.PREC_F64, bufs[0][0], bufs[0][1], kernel=I.KERNEL_SIMPLE)
+    I.synchronize()
+    timer = I.Timer()
+    timer.start()
+    for mt, (bb, xx) in zip(mats, bufs):
+        I.solve(mt, None, I.PREC_F64, bb, xx, kernel=I.KERNEL_SIMPLE)
+    ms_seq = timer.stop()
+    for mt in mats:
+        assert I.check_watchdog(mt) == 0
+        I.free_device_analysis(mt, None)
+    for bb, xx in bufs:
+        bb.free()
+        xx.free()
+    log(f"[bench] FP64 batched one launch: {ms64b:.3f} ms; FP64 sequential: {ms_seq:.1f} ms")
+    bytes_jac_rows = sum(sm.nnz for m, sm in enumerate(systems) if kinds[m] == I.KERNEL_JACOBI)
+    out = {
+        "metric": METRIC,
+        "value": round(nnz / (msm * 1e-3) / 1e9, 4),
+        "unit": "GNNZ/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(msm, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {
+            "workload": desc, "systems": nmat, "rows": rows, "nnz": nnz,
+            "jacobi_members": kinds.count(I.KERNEL_JACOBI),
+            "jacobi_schedule": {"sweeps": jc.sweeps, "outer": jc.outer} if jc else None,
+            "jacobi_nnz_fraction": round(bytes_jac_rows / nnz, 4),
+            "rel_err_mixed_max_member": max(errs),
+            "f64_batched_one_launch_ms": round(ms64b, 3),
+            "f64_sequential_ms": round(ms_seq, 2),
+            "speedup_vs_sequential_f64": round(ms_seq / msm, 2),
+            "speedup_vs_batched_f64": round(ms64b / msm, 3),
+            "kernel_ms_sum_per_solve": round(kms / max(args.steps, 1), 4), "kernel_launches": launches,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    db.free()
+    dx.free()
+    bt.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", type=int, default=2, choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", type=int, default=2, choices=sorted(WORKLOADS),
+                    help="2 (default, the headline), 3, 4 (16 RHS) or 5 (batch of 256 systems)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -128,6 +227,8 @@ def main() -> int:
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as O
 
+    if args.workload == 5:
+        return run_batch(args, I, O)
     desc, param = WORKLOADS[args.workload]
     t0 = time.time()
     s = I.generate(args.workload, param)
@@ -188,6 +289,30 @@ def main() -> int:
     log(f"[bench] plain FP64: {ms64:.3f} ms/solve (kernel {kms64:.3f} ms) rel err {err64:.2e}")
     msm, kmsm, errm = run(I.PREC_MIXED)
     log(f"[bench] FP32+refine: {msm:.3f} ms/solve (kernel {kmsm:.3f} ms) rel err {errm:.2e}")
+    per_rhs = None
+    if k > 1:
+        # the plain FP64 sync-free solve one right-hand side at a time (k solves)
+        b1 = I.DeviceArray.from_numpy(np.ascontiguousarray(s.b.reshape(s.n, k)[:, 0]))
+        x1 = I.DeviceArray(s.n, np.float64)
+        c1, _ = (I.default_config(mat, info, I.PREC_F64, 1), 0.0) if args.no_tune else \
+            I.autotune(mat, info, I.PREC_F64, k=1, reps=3)
+        for _ in range(args.warmup):
+            I.solve(mat, info, I.PREC_F64, b1, x1, k=1, cfg=c1)
+        I.synchronize()
+        timer = I.Timer()
+        timer.start()
+        for _ in range(args.steps):
+            I.solve(mat, info, I.PREC_F64, b1, x1, k=1, cfg=c1)
+        t1 = timer.stop() / args.steps
+        assert I.check_watchdog(mat) == 0
+        ref1 = O.fwd_f64(s.ia, s.ja, s.a, np.ascontiguousarray(s.b.reshape(s.n, k)[:, 0]))
+        e1 = float(np.max(np.abs(x1.numpy() - ref1)) / np.max(np.abs(ref1)))
+        per_rhs = {"ms_per_rhs": round(t1, 4), "ms_all_rhs": round(t1 * k, 4),
+                   "gnnz_s": round(s.nnz / (t1 * 1e-3) / 1e9, 4), "kernel": cfg_dict(c1), "rel_err": e1,
+                   "speedup_mixed_vs_per_rhs_f64": round(t1 * k / msm, 3)}
+        log(f"[bench] plain FP64 per RHS: {t1:.3f} ms x {k} = {t1 * k:.3f} ms")
+        b1.free()
+        x1.free()
     cold64, coldm = cold(I.PREC_F64), cold(I.PREC_MIXED)
     tdep = I.dep_latency_ns(2000)
     log(f"[bench] cold-cache: FP64 {cold64:.3f} ms, mixed {coldm:.3f} ms; t_dep {tdep:.0f} ns one-way")
@@ -265,6 +390,7 @@ def main() -> int:
             "latency_bound_fraction_f64": round(nlevs * tdep * 1e-6 / ms64, 3),
             "power_w": power,
             "rocsparse_csrsv": rocsparse,
+            "plain_f64_per_rhs": per_rhs,
         },
         "roofline": {
             "bound": "hbm",

@@ -270,6 +270,9 @@ int sptrsv_batch_solve_graph(sptrsv_batch_t* bt, int precision, const double* b,
                              double* x, hipStream_t stream);
 int sptrsv_batch_row_offsets(const sptrsv_batch_t* bt, int64_t* row_off /* nmat+1 */);
 int sptrsv_batch_choice(const sptrsv_batch_t* bt, int m, sptrsv_config_t* cfg);
+/* sptrsv_check for the batch: watchdog, or a Jacobi member that missed its
+ * certified bound (each member is certified separately). */
+int sptrsv_batch_check(sptrsv_batch_t* bt, hipStream_t stream);
 void sptrsv_batch_destroy(sptrsv_batch_t* bt);
 
 /* ---- vendor baseline (replaces csr_L_solve_cusparse_v2, solver.h:8) ---- */

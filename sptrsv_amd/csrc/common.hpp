@@ -62,7 +62,14 @@ struct MatPriv {
     float* jac_a = nullptr;      // Jacobi iterates (ping-pong), n*k each
     float* jac_b = nullptr;
     size_t jac_len = 0;
-    unsigned long long* jac_norms = nullptr;  // max |r|, max |x| (double bits)
+    unsigned long long* jac_norms = nullptr;  // per-member max |r|, max |x| (double bits), replicated
+    size_t jac_norms_len = 0;    // in 64-bit words
+    int* jac_tiles = nullptr;    // row tiles, 4 ints each: first row, end row, first entry, end entry
+    int jac_ntiles = 0;
+    int jac_tiles_K = 0;         // right-hand-side width the tiles were cut for
+    bool jac_tiles_fixed = false;  // tiles installed by the caller (batch), never rebuilt
+    int jac_members = 1;         // independent systems sharing the certificate (batch)
+    int* jac_tile_member = nullptr;  // member of each tile (device; null: one member)
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};
     int tuned_k[3] = {0, 0, 0};

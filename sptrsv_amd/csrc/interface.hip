@@ -81,6 +81,8 @@ static void free_mat(sp_mat_t* m) {
         for (float* p : {pv->jac_r, pv->jac_a, pv->jac_b})
             if (p) (void)hipFree(p);
         if (pv->jac_norms) (void)hipFree(pv->jac_norms);
+        if (pv->jac_tiles) (void)hipFree(pv->jac_tiles);
+        if (pv->jac_tile_member) (void)hipFree(pv->jac_tile_member);
         delete pv;
     }
     memset(m, 0, sizeof(*m));

@@ -12,13 +12,25 @@
 // HBM-bound passes.
 //
 // One refinement step (x in FP64):
-//   r = b - L x (FP64), r32 = (float) r, d_0 = r32 / diag       [k_jac_residual]
+//   r = b - L x (FP64), r32 = (float) r, d_0 = r32 / diag       [k_jac_resid / k_jac_first]
 //   d_s = (r32 - N d_{s-1}) / diag, s = 1 .. m-1 (FP32)          [k_jac_sweep]
 //   x += d_{m-1} (FP64), fused into the last sweep
 // Certificate: with delta = min_i(|L_ii| - sum_j |L_ij|) > 0, |L^-1|_inf <=
-// 1/delta, so |x - x*|_inf <= |b - L x|_inf / delta; the final residual pass
-// takes the max-norms and a one-thread kernel compares the bound with
-// tol * |x|_inf, leaving the verdict in a device word (no host round trip).
+// 1/delta, so |x - x*|_inf <= |b - L x|_inf / delta.  A final residual pass
+// takes max |r| and max |x| per independent system (one, or each member of a
+// batch) and k_jac_certify compares the bound with tol * |x|_inf, leaving the
+// verdict in a device word (no host round trip).
+//
+// Kernel layout (both passes): rows are cut into tiles by nnz windows -- tile
+// t holds the rows whose first entry lies in [t*W, (t+1)*W), W = 1024/K
+// entries for K right-hand sides -- so every tile streams about the same
+// number of entries.  A 256-thread workgroup stages its tile's row pointers,
+// diagonal and right-hand side in LDS, streams the entries with every lane
+// busy and all loads of a lane issued before their first use (phase 1: a
+// product per (entry, rhs) into LDS), then sums rows out of LDS with a lane
+// group per row sized to the tile's mean row length (phase 2).  A tile whose
+// products do not fit LDS (a very long row) sums straight from memory with a
+// wavefront per row.
 #include <math.h>
 
 #include <algorithm>
@@ -28,19 +40,30 @@
 
 namespace sptrsv {
 
-struct JacArgs {
+constexpr int kTileCap = 2048;   // products staged in LDS per tile
+constexpr int kTileThreads = 256;
+constexpr int kNormReps = 16;    // replicas of each member's norm words (spread atomics)
+constexpr int kNormStride = 16;  // 64-bit words per replica (one 128-byte line)
+
+template <int K>
+__host__ __device__ constexpr int tile_window() {
+    return 1024 / K;
+}
+
+struct TileArgs {
     int n;
     int k;
     const int* ia;
     const int* ja;
-    const double* a64;
     const float* a32;
-    const double* b;
-    double* x;
-    float* r32;
-    int L;
-    int lgL;
-    unsigned long long* norms;  // [0] max |r| bits, [1] max |x| bits (non-negative doubles)
+    const double* a64;
+    const int4* tiles;        // per tile {first row, end row, first entry, end entry}
+    const int* tile_member;   // member (independent system) of each tile; null: one member
+};
+
+struct NormArgs {
+    unsigned long long* norms;  // null: no norms; else [member][rep][stride]: [0] |r|, [1] |x|
+    int members;
 };
 
 template <class T>
@@ -54,85 +77,276 @@ __device__ __forceinline__ T jac_group_sum(T v, int G, int K) {
     return v;
 }
 
-// r = b - L x over the row (diagonal included) in FP64; x == nullptr means
-// x = 0.  Writes r32 and the first iterate d0 = r32 / diag; with norms set,
-// also folds max |r_i| and max |x_i| into norms.
+// Lane-group shape for phase 2: G2 lanes per row and right-hand side,
+// the smallest power of two covering the mean row length (at most 64 / K).
 template <int K>
-__global__ __launch_bounds__(256) void k_jac_residual(JacArgs A, float* d0, bool with_norms) {
-    const int lgG = A.lgL + kLog2<K>();
-    const int G = 1 << lgG;
-    const int lane = threadIdx.x & 63;
-    const int gl = lane & (G - 1);
-    const int e = gl / K, rr = gl % K;
-    const long long row = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> lgG;
-    const bool live = row < A.n && rr < A.k;
-    double s = 0.0;
-    int re = 0;
-    if (live) {
-        const int rs = A.ia[row];
-        re = A.ia[row + 1];
-        if (A.x)
-            for (int p = rs + e; p < re; p += A.L)
-                s = __fma_rn(A.a64[p], A.x[(size_t)A.ja[p] * A.k + rr], s);
+__device__ __forceinline__ int group_lanes(int E, int rows) {
+    const int avg = (E + rows - 1) / rows;
+    int g = 1;
+    while (g < avg && g * K < kWave) g <<= 1;
+    return g;
+}
+
+// Stages one tile in LDS with every independent load issued before the
+// first use: level 1 the tile bounds; level 2 the row pointers, the
+// per-row right-hand side (V: FP32 residual or FP64 b) and the entries;
+// level 3 the diagonals and the gathered iterate; then all LDS stores.
+// Products (a * v[ja], diagonal included) go to prod when the tile fits.
+template <int K, class V, class P>
+struct TileStage {
+    static constexpr int W = tile_window<K>();
+    static constexpr int RU = (W + 1 + kTileThreads - 1) / kTileThreads;
+    static constexpr int SU = (W * K + kTileThreads - 1) / kTileThreads;
+    static constexpr int U = kTileCap / kTileThreads;
+    int r0, rows, p0, E;
+    bool lds;
+
+    __device__ bool run(const TileArgs& A, const V* rhs, const P* vals, const P* v, int* s_ia, float* s_dg,
+                        V* s_rhs, P* prod) {
+        const int tid = threadIdx.x;
+        const int k = A.k;
+        const int4 tl = A.tiles[blockIdx.x];
+        r0 = tl.x;
+        rows = tl.y - tl.x;
+        p0 = tl.z;
+        E = tl.w - tl.z;
+        if (rows <= 0) return false;
+        lds = E * K <= kTileCap;
+        // level 2
+        int iav[RU];
+#pragma unroll
+        for (int u = 0; u < RU; ++u)
+            iav[u] = (u * kTileThreads <= rows) ? A.ia[r0 + min(tid + u * kTileThreads, rows)] : 1;
+        V rv[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            rv[u] = V(0);
+            if (u * kTileThreads < rows * K) {
+                const int t = tid + u * kTileThreads;
+                const int lr = min(t / K, rows - 1), rr = min(t & (K - 1), k - 1);
+                rv[u] = rhs[(size_t)(r0 + lr) * k + rr];
+            }
+        }
+        int jv[U];
+        P av[U];
+        if (lds) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                jv[u] = 0;
+                av[u] = P(0);
+                if (u * kTileThreads < E * K) {
+                    const int e = min((tid + u * kTileThreads) / K, E - 1);
+                    jv[u] = A.ja[p0 + e];
+                    av[u] = vals[p0 + e];
+                }
+            }
+        }
+        // level 3
+        float dg[RU];
+#pragma unroll
+        for (int u = 0; u < RU; ++u) dg[u] = (u * kTileThreads <= rows) ? A.a32[max(iav[u] - 1, 0)] : 1.0f;
+        P xv[U];
+        if (lds) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                xv[u] = P(0);
+                if (u * kTileThreads < E * K) {
+                    const int rr = min((tid + u * kTileThreads) & (K - 1), k - 1);
+                    xv[u] = v[(size_t)jv[u] * k + rr];
+                }
+            }
+        }
+        // LDS
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const int t = tid + u * kTileThreads;
+            if (t <= rows) s_ia[t] = iav[u] - p0;
+            if (t >= 1 && t <= rows) s_dg[t - 1] = dg[u];
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int t = tid + u * kTileThreads;
+            if (t < rows * K) s_rhs[t] = (t & (K - 1)) < k ? rv[u] : V(0);
+        }
+        if (lds) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = tid + u * kTileThreads;
+                if (t < E * K) prod[t] = av[u] * xv[u];
+            }
+        }
+        return true;
     }
-    s = jac_group_sum(s, G, K);
-    if (live && e == 0) {
-        const size_t i = (size_t)row * A.k + rr;
-        const double r = A.b[i] - s;
-        const float r32 = (float)r;
-        A.r32[i] = r32;
-        if (d0) d0[i] = r32 * (1.0f / A.a32[re - 1]);
-        if (with_norms) {
-            atomicMax(&A.norms[0], (unsigned long long)__double_as_longlong(fabs(r)));
-            if (A.x)
-                atomicMax(&A.norms[1], (unsigned long long)__double_as_longlong(fabs(A.x[i])));
+};
+
+// One Jacobi sweep: d_new = (r32 - N d_old) / diag.  mode 0 stores d_new,
+// mode 1 sets x = d_new (first refinement step, x was 0), mode 2 adds
+// x += d_new.
+template <int K>
+__global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const float* __restrict__ r32,
+                                                            const float* __restrict__ d_old,
+                                                            float* __restrict__ d_new, double* x, int mode) {
+    constexpr int W = tile_window<K>();
+    __shared__ float prod[kTileCap];
+    __shared__ int s_ia[W + 1];
+    __shared__ float s_dg[W];
+    __shared__ float s_r[W * K];
+    TileStage<K, float, float> T;
+    if (!T.run(A, r32, A.a32, d_old, s_ia, s_dg, s_r, prod)) return;
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const int k = A.k;
+    // phase 2: a group of G2*K lanes per row
+    const int G2 = group_lanes<K>(T.E, T.rows);
+    const int GL = G2 * K;
+    const int rpp = kTileThreads / GL;
+    const int npass = (T.rows + rpp - 1) / rpp;
+    const int gl = tid % GL, e = gl / K, rr = gl & (K - 1);
+    for (int ps = 0; ps < npass; ++ps) {
+        const int lr = ps * rpp + tid / GL;
+        const bool live = lr < T.rows && rr < k;
+        float s = 0.0f;
+        if (live) {
+            const int rs = s_ia[lr], re = s_ia[lr + 1] - 1;  // diagonal excluded
+            if (T.lds) {
+                for (int q = rs + e; q < re; q += G2) s += prod[q * K + rr];
+            } else {
+#pragma unroll 4
+                for (int q = rs + e; q < re; q += G2)
+                    s = __fmaf_rn(A.a32[T.p0 + q], d_old[(size_t)A.ja[T.p0 + q] * k + rr], s);
+            }
+        }
+        s = jac_group_sum(s, GL, K);
+        if (live && e == 0) {
+            const size_t i = (size_t)(T.r0 + lr) * k + rr;
+            const float d = (s_r[lr * K + rr] - s) * (1.0f / s_dg[lr]);
+            if (mode == 0)
+                d_new[i] = d;
+            else if (mode == 1)
+                x[i] = (double)d;
+            else
+                x[i] += (double)d;
         }
     }
 }
 
-// d_new = (r32 - N d_old) / diag in FP32.  mode 0 stores d_new, mode 1 sets
-// x = d_new (first refinement step, x was 0), mode 2 adds x += d_new.
+__device__ __forceinline__ unsigned long long* norm_slot(const NormArgs& N, int member, int word) {
+    return N.norms + ((size_t)member * kNormReps + blockIdx.x % kNormReps) * kNormStride + word;
+}
+
+// r = b - L x in FP64 (diagonal included).  Writes r32 = (float) r when
+// r32_out is set, d0 = r32 / diag when d0 is set, and with N.norms folds
+// max |r| and max |x| per member.
 template <int K>
-__global__ __launch_bounds__(256) void k_jac_sweep(JacArgs A, const float* __restrict__ d_old,
-                                                   float* __restrict__ d_new, int mode) {
-    const int lgG = A.lgL + kLog2<K>();
-    const int G = 1 << lgG;
-    const int lane = threadIdx.x & 63;
-    const int gl = lane & (G - 1);
-    const int e = gl / K, rr = gl % K;
-    const long long row = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> lgG;
-    const bool live = row < A.n && rr < A.k;
-    float s = 0.0f;
-    int re = 0;
-    if (live) {
-        const int rs = A.ia[row];
-        re = A.ia[row + 1];
-        for (int p = rs + e; p < re - 1; p += A.L)
-            s = __fmaf_rn(A.a32[p], d_old[(size_t)A.ja[p] * A.k + rr], s);
+__global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const double* __restrict__ b,
+                                                            const double* __restrict__ x,
+                                                            float* __restrict__ r32_out,
+                                                            float* __restrict__ d0, NormArgs N) {
+    constexpr int W = tile_window<K>();
+    __shared__ double prod[kTileCap];
+    __shared__ int s_ia[W + 1];
+    __shared__ float s_dg[W];
+    __shared__ double s_b[W * K];
+    __shared__ unsigned long long s_nrm[2];
+    TileStage<K, double, double> T;
+    if (!T.run(A, b, A.a64, x, s_ia, s_dg, s_b, prod)) return;
+    const int tid = threadIdx.x;
+    const int k = A.k;
+    if (N.norms && tid < 2) s_nrm[tid] = 0ull;
+    __syncthreads();
+    const int G2 = group_lanes<K>(T.E, T.rows);
+    const int GL = G2 * K;
+    const int rpp = kTileThreads / GL;
+    const int npass = (T.rows + rpp - 1) / rpp;
+    const int gl = tid % GL, e = gl / K, rr = gl & (K - 1);
+    for (int ps = 0; ps < npass; ++ps) {
+        const int lr = ps * rpp + tid / GL;
+        const bool live = lr < T.rows && rr < k;
+        double s = 0.0;
+        if (live) {
+            const int rs = s_ia[lr], re = s_ia[lr + 1];
+            if (T.lds) {
+                for (int q = rs + e; q < re; q += G2) s += prod[q * K + rr];
+            } else {
+#pragma unroll 4
+                for (int q = rs + e; q < re; q += G2)
+                    s = __fma_rn(A.a64[T.p0 + q], x[(size_t)A.ja[T.p0 + q] * k + rr], s);
+            }
+        }
+        s = jac_group_sum(s, GL, K);
+        if (live && e == 0) {
+            const size_t i = (size_t)(T.r0 + lr) * k + rr;
+            const double r = s_b[lr * K + rr] - s;
+            const float rf = (float)r;
+            if (r32_out) r32_out[i] = rf;
+            if (d0) d0[i] = rf * (1.0f / s_dg[lr]);
+            if (N.norms) {
+                atomicMax(&s_nrm[0], (unsigned long long)__double_as_longlong(fabs(r)));
+                atomicMax(&s_nrm[1], (unsigned long long)__double_as_longlong(fabs(x[i])));
+            }
+        }
     }
-    s = jac_group_sum(s, G, K);
-    if (live && e == 0) {
-        const size_t i = (size_t)row * A.k + rr;
-        const float d = (A.r32[i] - s) * (1.0f / A.a32[re - 1]);
-        if (mode == 0)
-            d_new[i] = d;
-        else if (mode == 1)
-            A.x[i] = (double)d;
-        else
-            A.x[i] += (double)d;
+    if (N.norms) {
+        __syncthreads();
+        if (tid < 2 && s_nrm[tid] != 0ull) {
+            const int m = A.tile_member ? A.tile_member[blockIdx.x] : 0;
+            atomicMax(norm_slot(N, m, tid), s_nrm[tid]);
+        }
     }
 }
 
-__global__ void k_jac_zero_norms(unsigned long long* norms) {
-    if (threadIdx.x < 2) norms[threadIdx.x] = 0ull;
+// First refinement step (x = 0): r32 = (float) b, d0 = r32 / diag.
+template <int K>
+__global__ __launch_bounds__(256) void k_jac_first(TileArgs A, const double* __restrict__ b, float* r32,
+                                                   float* d0) {
+    constexpr int lgK = kLog2<K>();
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long row = idx >> lgK;
+    const int rr = (int)(idx & (K - 1));
+    if (row >= A.n || rr >= A.k) return;
+    const size_t i = (size_t)row * A.k + rr;
+    const float rf = (float)b[i];
+    r32[i] = rf;
+    d0[i] = rf * (1.0f / A.a32[A.ia[row + 1] - 1]);
 }
 
-// status = 0 when |r|_inf / delta <= tol * |x|_inf, else 1.
-__global__ void k_jac_certify(const unsigned long long* norms, double delta, double tol, int* status) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double rn = __longlong_as_double((long long)norms[0]);
-        const double xn = __longlong_as_double((long long)norms[1]);
-        const bool ok = rn / delta <= tol * xn || rn == 0.0;
+// Row tiles by nnz windows: tile t = {first row whose first entry is >=
+// t * window, the same for t + 1 (n for the last), their first entries}.
+__device__ __forceinline__ int first_row_at(const int* ia, int n, long long v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ia[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_jac_tiles(const int* ia, int n, int window, int ntiles, int4* tiles) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int r0 = first_row_at(ia, n, (long long)t * window);
+    const int r1 = (t + 1 == ntiles) ? n : first_row_at(ia, n, (long long)(t + 1) * window);
+    tiles[t] = make_int4(r0, r1, ia[r0], ia[r1]);
+}
+
+__global__ void k_jac_zero_norms(unsigned long long* norms, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        norms[i] = 0ull;
+}
+
+// status = 1 unless, for every member, |r|_inf / delta <= tol * |x|_inf.
+__global__ void k_jac_certify(const unsigned long long* norms, int members, double delta, double tol,
+                              int* status) {
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < members; m += gridDim.x * blockDim.x) {
+        unsigned long long rb = 0, xb = 0;
+        for (int rep = 0; rep < kNormReps; ++rep) {
+            const unsigned long long* s = norms + ((size_t)m * kNormReps + rep) * kNormStride;
+            rb = max(rb, s[0]);
+            xb = max(xb, s[1]);
+        }
+        const double rn = __longlong_as_double((long long)rb);
+        const double xn = __longlong_as_double((long long)xb);
+        const bool ok = rn == 0.0 || rn / delta <= tol * xn;
         if (!ok) *status = 1;
     }
 }
@@ -168,11 +382,6 @@ __global__ void k_fill_hash(double* b, size_t count) {
     }
 }
 
-static unsigned rows_grid(long long n, int lanes_total) {
-    long long threads = n * lanes_total;
-    return (unsigned)std::max<long long>(1, (threads + 255) / 256);
-}
-
 int jacobi_margin(sp_mat_t* m, double* delta) {
     MatPriv* pv = priv_of(m);
     if (pv->margin_known) {
@@ -197,6 +406,14 @@ int jacobi_margin(sp_mat_t* m, double* delta) {
     return SPTRSV_OK;
 }
 
+static int pow2_at_least(int k) {
+    int K = 1;
+    while (K < k) K <<= 1;
+    return K;
+}
+
+// Workspaces and the tile table for k right-hand sides (allocates; call
+// before any capture).
 static int ensure_jacobi_ws(sp_mat_t* m, int k) {
     MatPriv* pv = priv_of(m);
     const size_t words = std::max<size_t>((size_t)m->nr * k, 1);
@@ -210,11 +427,83 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
         SPT_HIP(hipMalloc(&pv->jac_b, words * sizeof(float)));
         pv->jac_len = words;
     }
-    if (!pv->jac_norms) SPT_HIP(hipMalloc(&pv->jac_norms, 2 * sizeof(unsigned long long)));
+    const size_t nw = (size_t)std::max(1, pv->jac_members) * kNormReps * kNormStride;
+    if (pv->jac_norms_len < nw) {
+        if (pv->jac_norms) SPT_HIP(hipFree(pv->jac_norms));
+        pv->jac_norms = nullptr;
+        pv->jac_norms_len = 0;
+        SPT_HIP(hipMalloc(&pv->jac_norms, nw * sizeof(unsigned long long)));
+        pv->jac_norms_len = nw;
+    }
+    const int K = pow2_at_least(k);
+    if (pv->jac_tiles_K != K) {
+        if (pv->jac_tiles_fixed) return fail(SPTRSV_E_CAPACITY, "batched JACOBI solves take one right-hand side");
+        if (pv->jac_tiles) SPT_HIP(hipFree(pv->jac_tiles));
+        pv->jac_tiles = nullptr;
+        pv->jac_tiles_K = 0;
+        const int window = 1024 / K;
+        const int ntiles = (int)std::max<long long>(1, ((long long)m->nnz + window - 1) / window);
+        SPT_HIP(hipMalloc(&pv->jac_tiles, 4 * sizeof(int) * (size_t)ntiles));
+        hipLaunchKernelGGL(k_jac_tiles, dim3(ceil_div(ntiles, 256)), dim3(256), 0, nullptr, m->ia, m->nr,
+                           window, ntiles, reinterpret_cast<int4*>(pv->jac_tiles));
+        SPT_HIP(hipGetLastError());
+        SPT_HIP(hipDeviceSynchronize());
+        pv->jac_ntiles = ntiles;
+        pv->jac_tiles_K = K;
+    }
     return SPTRSV_OK;
 }
 
 int reserve_jacobi(sp_mat_t* m, int k) { return ensure_jacobi_ws(m, k); }
+
+// Host-side tiles of rows [r_begin, r_end) of a CSR whose row pointers are
+// ia (relative: ia[r] - ia[r_begin] is the entry offset inside the range),
+// appended as {first row, end row, first entry, end entry} with rows and
+// entries shifted by row_base / entry_base.  Same windows as k_jac_tiles,
+// restarted at the range's first entry, so tiles never straddle ranges.
+void jacobi_host_tiles(const int* ia, int r_begin, int r_end, long long row_base, long long entry_base,
+                       std::vector<int>& out) {
+    const int window = 1024;
+    const long long e0 = ia[r_begin];
+    const long long nz = (long long)ia[r_end] - e0;
+    int r = r_begin;
+    for (long long t = 0; t * window < nz; ++t) {
+        const int first = r;
+        while (r < r_end && (long long)ia[r] - e0 < (t + 1) * window) ++r;
+        if (r == first) continue;  // window inside one long row
+        out.push_back((int)(row_base + first - r_begin));
+        out.push_back((int)(row_base + r - r_begin));
+        out.push_back((int)(entry_base + ia[first] - e0));
+        out.push_back((int)(entry_base + ia[r] - e0));
+    }
+}
+
+// Installs a tile table built on the host (one right-hand side) covering the
+// rows the JACOBI path should solve, with the member of each tile; members
+// without tiles are left to other solvers.  delta is the dominance margin of
+// the covered rows.
+int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>& tiles4, const std::vector<int>& tile_member,
+                     int members, double delta) {
+    MatPriv* pv = priv_of(m);
+    if (pv->jac_tiles) SPT_HIP(hipFree(pv->jac_tiles));
+    if (pv->jac_tile_member) SPT_HIP(hipFree(pv->jac_tile_member));
+    pv->jac_tiles = nullptr;
+    pv->jac_tile_member = nullptr;
+    pv->jac_ntiles = 0;
+    pv->jac_tiles_K = 1;
+    pv->jac_tiles_fixed = true;
+    pv->jac_members = std::max(1, members);
+    pv->margin = delta;
+    pv->margin_known = true;
+    const size_t nt = tile_member.size();
+    if (nt == 0) return SPTRSV_OK;
+    SPT_HIP(hipMalloc(&pv->jac_tiles, sizeof(int) * tiles4.size()));
+    SPT_HIP(hipMemcpy(pv->jac_tiles, tiles4.data(), sizeof(int) * tiles4.size(), hipMemcpyHostToDevice));
+    SPT_HIP(hipMalloc(&pv->jac_tile_member, sizeof(int) * nt));
+    SPT_HIP(hipMemcpy(pv->jac_tile_member, tile_member.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+    pv->jac_ntiles = (int)nt;
+    return SPTRSV_OK;
+}
 
 hipEvent_t* kernel_timer_pair(hipStream_t s);
 
@@ -222,61 +511,51 @@ template <int K>
 static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, double* x,
                            hipStream_t s) {
     MatPriv* pv = priv_of(m);
-    const int L = std::max(1, std::min(c.lanes, kWave / K));
-    int lgL = 0;
-    while ((1 << (lgL + 1)) <= L) ++lgL;
-    JacArgs A{};
-    A.n = m->nr;
-    A.k = k;
-    A.ia = m->ia;
-    A.ja = m->ja;
-    A.a64 = m->a;
-    A.a32 = m->a32;
-    A.b = b;
-    A.x = nullptr;
-    A.r32 = pv->jac_r;
-    A.L = L;
-    A.lgL = lgL;
-    A.norms = pv->jac_norms;
-    const unsigned grid = rows_grid(m->nr, L * K);
+    TileArgs A{m->nr, k, m->ia, m->ja, m->a32, m->a, reinterpret_cast<const int4*>(pv->jac_tiles),
+               pv->jac_tile_member};
+    const NormArgs none{nullptr, 1};
+    const NormArgs with{pv->jac_norms, std::max(1, pv->jac_members)};
+    if (pv->jac_ntiles == 0) return SPTRSV_OK;
+    const unsigned tiles = (unsigned)pv->jac_ntiles;
+    const unsigned first_grid = ceil_div((long long)m->nr * K, 256);
     const int sweeps = std::max(2, c.sweeps);
     const int outer = std::max(1, c.outer);
     for (int o = 0; o < outer; ++o) {
-        A.x = (o == 0) ? nullptr : x;
-        hipLaunchKernelGGL((k_jac_residual<K>), dim3(grid), dim3(256), 0, s, A, pv->jac_a, false);
+        if (o == 0)
+            hipLaunchKernelGGL((k_jac_first<K>), dim3(first_grid), dim3(256), 0, s, A, b, pv->jac_r, pv->jac_a);
+        else
+            hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b,
+                               (const double*)x, pv->jac_r, pv->jac_a, none);
         float* cur = pv->jac_a;
         float* nxt = pv->jac_b;
-        A.x = x;
         for (int sw = 1; sw < sweeps; ++sw) {
             const bool last = sw == sweeps - 1;
             const int mode = last ? (o == 0 ? 1 : 2) : 0;
             hipEvent_t* ev = kernel_timer_pair(s);
             if (ev) SPT_HIP(hipEventRecord(ev[0], s));
-            hipLaunchKernelGGL((k_jac_sweep<K>), dim3(grid), dim3(256), 0, s, A, cur, nxt, mode);
+            hipLaunchKernelGGL((k_jac_sweep<K>), dim3(tiles), dim3(kTileThreads), 0, s, A,
+                               (const float*)pv->jac_r, (const float*)cur, nxt, x, mode);
             if (ev) SPT_HIP(hipEventRecord(ev[1], s));
             std::swap(cur, nxt);
         }
     }
-    // certificate: final FP64 residual with max-norms
-    hipLaunchKernelGGL(k_jac_zero_norms, dim3(1), dim3(64), 0, s, pv->jac_norms);
-    A.x = x;
-    hipLaunchKernelGGL((k_jac_residual<K>), dim3(grid), dim3(256), 0, s, A, (float*)nullptr, true);
-    hipLaunchKernelGGL(k_jac_certify, dim3(1), dim3(64), 0, s, pv->jac_norms, pv->margin, pv->tol,
-                       pv->counters + 3);
+    // certificate: final FP64 residual with per-member max-norms
+    const size_t nw = (size_t)with.members * kNormReps * kNormStride;
+    hipLaunchKernelGGL(k_jac_zero_norms, dim3(ceil_div((long long)nw, 256)), dim3(256), 0, s, pv->jac_norms, nw);
+    hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b, (const double*)x,
+                       (float*)nullptr, (float*)nullptr, with);
+    hipLaunchKernelGGL(k_jac_certify, dim3(ceil_div(with.members, 256)), dim3(256), 0, s, pv->jac_norms,
+                       with.members, pv->margin, pv->tol, pv->counters + 3);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }
 
 int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, double* x, hipStream_t s) {
-    MatPriv* pv = priv_of(m);
     double delta = 0;
     SPT_TRY(jacobi_margin(m, &delta));
     if (!(delta > 0)) return fail(SPTRSV_E_INVALID_ARG, "JACOBI needs a strictly diagonally dominant matrix");
     SPT_TRY(ensure_jacobi_ws(m, k));
-    (void)pv;
-    int K = 1;
-    while (K < k) K <<= 1;
-    switch (K) {
+    switch (pow2_at_least(k)) {
         case 1: return jacobi_schedule<1>(m, c, k, b, x, s);
         case 2: return jacobi_schedule<2>(m, c, k, b, x, s);
         case 4: return jacobi_schedule<4>(m, c, k, b, x, s);
@@ -302,50 +581,44 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
     SPT_HIP(hipMalloc(&x, words * sizeof(double)));
     hipLaunchKernelGGL(k_fill_hash, dim3(1024), dim3(256), 0, nullptr, b, words);
     const double tol = pv->tol;
-    int K = 1;
-    while (K < k) K <<= 1;
-    const int avg = m->nr ? m->nnz / m->nr : 1;
-    int lanes = 1;
-    while (lanes < avg) lanes <<= 1;
-    lanes = std::min(lanes, kWave / K);
     hipEvent_t e0, e1;
     SPT_HIP(hipEventCreate(&e0));
     SPT_HIP(hipEventCreate(&e1));
     float bt = 1e30f;
     sptrsv_config_t bc{};
     int st = SPTRSV_OK;
-    for (int sweeps : {4, 6, 8, 12, 16}) {
-        for (int L : {std::max(1, lanes / 2), lanes}) {
-            sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, L, 256, 1, 0, 0, sweeps, 1};
-            // find the smallest number of refinement steps that certifies tol/8
-            bool found = false;
-            pv->tol = tol / 8;
-            for (int outer = 1; outer <= 16 && st == SPTRSV_OK; ++outer) {
-                c.outer = outer;
-                int zero = 0;
-                SPT_HIP(hipMemcpy(pv->counters + 3, &zero, sizeof(int), hipMemcpyHostToDevice));
-                st = jacobi_solve(m, c, k, b, x, nullptr);
-                int flag = 0;
-                SPT_HIP(hipMemcpy(&flag, pv->counters + 3, sizeof(int), hipMemcpyDeviceToHost));
-                if (st == SPTRSV_OK && flag == 0) {
-                    found = true;
-                    break;
-                }
+    int prev_outer = 16;
+    for (int sweeps : {4, 6, 8, 12, 16, 24}) {
+        sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, 1, 256, 1, 0, 0, sweeps, 1};
+        // the smallest number of refinement steps that certifies tol/8
+        bool found = false;
+        pv->tol = tol / 8;
+        for (int outer = 1; outer <= std::min(16, prev_outer) && st == SPTRSV_OK; ++outer) {
+            c.outer = outer;
+            int zero = 0;
+            SPT_HIP(hipMemcpy(pv->counters + 3, &zero, sizeof(int), hipMemcpyHostToDevice));
+            st = jacobi_solve(m, c, k, b, x, nullptr);
+            int flag = 0;
+            SPT_HIP(hipMemcpy(&flag, pv->counters + 3, sizeof(int), hipMemcpyDeviceToHost));
+            if (st == SPTRSV_OK && flag == 0) {
+                found = true;
+                break;
             }
-            pv->tol = tol;
-            if (st != SPTRSV_OK) break;
-            if (!found) continue;
-            SPT_HIP(hipEventRecord(e0, nullptr));
-            for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
-            SPT_HIP(hipEventRecord(e1, nullptr));
-            SPT_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
-            ms /= reps;
-            if (ms < bt) {
-                bt = ms;
-                bc = c;
-            }
+        }
+        pv->tol = tol;
+        if (st != SPTRSV_OK) break;
+        if (!found) continue;
+        prev_outer = c.outer;  // more sweeps per step never need more steps
+        SPT_HIP(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
+        SPT_HIP(hipEventRecord(e1, nullptr));
+        SPT_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= std::max(reps, 1);
+        if (ms < bt) {
+            bt = ms;
+            bc = c;
         }
         if (st != SPTRSV_OK) break;
     }

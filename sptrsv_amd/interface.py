@@ -346,6 +346,10 @@ class Batch:
         fn = L.lib().sptrsv_batch_solve_graph if graph else L.lib().sptrsv_batch_solve
         check(fn(self.h, precision, b.ptr, x.ptr, stream), "sptrsv_batch_solve")
 
+    def check(self, stream=None) -> int:
+        """Status of the last solves (0, E_WATCHDOG or E_NOT_CONVERGED) without raising."""
+        return L.lib().sptrsv_batch_check(self.h, stream)
+
     def close(self) -> None:
         if self.h is not None:
             L.lib().sptrsv_batch_destroy(self.h)
