@@ -90,8 +90,8 @@ def cfg_dict(c) -> dict:
     if c.kernel == 4:
         d.update(sweeps=c.sweeps, outer=c.outer)
     else:
-        d.update(fused=c.fused, variant="one-row-per-wave" if c.variant else "multi-row-wave",
-                 cu_waves=c.cu_waves)
+        d.update(fused=c.fused, variant={0: "multi-row-wave", 1: "one-row-per-wave", 2: "persistent"}.get(
+            c.variant, c.variant), cu_waves=c.cu_waves, window=c.window)
     return d
 
 
@@ -218,6 +218,13 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--jacobi", type=int, nargs=2, metavar=("SWEEPS", "STEPS"),
+                    help="use the Jacobi-refinement mixed path with this schedule instead of tuning it")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the rocSPARSE, power and cold-cache legs (for profiler runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 5 plain device copies of 256 MiB (the library's k_copy16), the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this SpTRSV engine runs on one GPU; --gpus must be 1"}))
@@ -244,7 +251,9 @@ def main() -> int:
 
     tuned = {}
     for prec in (I.PREC_F64, I.PREC_MIXED):
-        if args.no_tune:
+        if prec == I.PREC_MIXED and args.jacobi:
+            tuned[prec] = I._cfg(I.KERNEL_JACOBI, 1, 256, 1, 0, 0, args.jacobi[0], args.jacobi[1])
+        elif args.no_tune:
             tuned[prec] = I.default_config(mat, info, prec, k)
         else:
             tuned[prec], _ = I.autotune(mat, info, prec, k=k, reps=3)
@@ -313,13 +322,15 @@ def main() -> int:
         log(f"[bench] plain FP64 per RHS: {t1:.3f} ms x {k} = {t1 * k:.3f} ms")
         b1.free()
         x1.free()
-    cold64, coldm = cold(I.PREC_F64), cold(I.PREC_MIXED)
+    cold64, coldm = (cold(I.PREC_F64), cold(I.PREC_MIXED)) if not args.no_extras else (float("nan"),) * 2
     tdep = I.dep_latency_ns(2000)
     log(f"[bench] cold-cache: FP64 {cold64:.3f} ms, mixed {coldm:.3f} ms; t_dep {tdep:.0f} ns one-way")
 
     # vendor baseline on the same device matrix (analysis timed separately)
     rocsparse = None
     try:
+        if args.no_extras:
+            raise RuntimeError("skipped (--no-extras)")
         rb = I.RocsparseBaseline(mat)
         rb.solve(db, dx)
         I.synchronize()
@@ -340,7 +351,7 @@ def main() -> int:
         rocsparse = {"error": str(e)[:200]}
 
     power = None
-    if I.power_start("bench", 0, 1000):
+    if not args.no_extras and I.power_start("bench", 0, 1000):
         for _ in range(max(args.steps, 50)):
             I.solve(mat, info, I.PREC_MIXED, db, dx, k=k, cfg=tuned[I.PREC_MIXED])
         I.synchronize()
@@ -384,7 +395,7 @@ def main() -> int:
             "bytes_mixed_method": int(b_method),
             "hbm_gbs_f64": round(bytes_f64(n, nnz, k) / (ms64 * 1e-3) / 1e9, 2),
             "bytes_method_separate_pass_convention": bytes_method_convention(n, nnz, k),
-            "cold_cache_ms": {"mixed": round(coldm, 4), "f64": round(cold64, 4)},
+            "cold_cache_ms": None if args.no_extras else {"mixed": round(coldm, 4), "f64": round(cold64, 4)},
             "t_dep_ns": round(tdep, 1),
             "latency_bound_fraction_mixed": round(nlevs * tdep * 1e-6 / msm, 3),
             "latency_bound_fraction_f64": round(nlevs * tdep * 1e-6 / ms64, 3),
@@ -409,6 +420,15 @@ def main() -> int:
     db.free()
     dx.free()
     I.free_device_analysis(mat, info)
+    if args.calibrate_copy:
+        src = I.DeviceArray(32 << 20, np.float64)                       # 256 MiB
+        dst = I.DeviceArray(32 << 20, np.float64)
+        for _ in range(5):
+            I.copy_device(dst, src, 256 << 20)
+        I.synchronize()
+        src.free()
+        dst.free()
+        log("[bench] calibration: 5 k_copy16 copies of 262144 KiB (read 262144 KiB, write 262144 KiB each)")
     return 0
 
 

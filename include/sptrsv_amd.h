@@ -194,6 +194,8 @@ typedef struct {
     int cu_waves;
     int sweeps;   /* JACOBI: FP32 sweeps per refinement step */
     int outer;    /* JACOBI: refinement steps */
+    int window;   /* sync-free: entries per lane kept in flight, 1, 4 or 16
+                     (16: one right-hand side, wg 256); 0 = by longest row */
 } sptrsv_config_t;
 
 /* Relative accuracy the JACOBI path certifies (default 1e-10). */
@@ -370,6 +372,9 @@ int sptrsv_memcpy_htod(void* dst, const void* src, size_t bytes);
 int sptrsv_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int sptrsv_memset(void* dst, int value, size_t bytes);
 int sptrsv_synchronize(void);
+/* Streaming device-to-device copy (bytes and pointers 16-byte aligned) with
+ * the library's own kernel: a known-size reference for traffic counters. */
+int sptrsv_copy_device(void* dst, const void* src, size_t bytes);
 /* Timed region helpers (hip events on `stream`): start records, stop
  * records and waits, returning milliseconds since start. */
 int sptrsv_timer_start(hipStream_t stream);

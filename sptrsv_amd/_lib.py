@@ -59,7 +59,7 @@ class AnalysisInfo(Structure):  # dfr_analysis_info_t, dfr_syncfree.h:25-41
 
 class Config(Structure):  # sptrsv_config_t
     _fields_ = [("kernel", c_int), ("lanes", c_int), ("wg", c_int), ("fused", c_int), ("variant", c_int),
-                ("cu_waves", c_int), ("sweeps", c_int), ("outer", c_int)]
+                ("cu_waves", c_int), ("sweeps", c_int), ("outer", c_int), ("window", c_int)]
 
 
 class BenchResult(Structure):  # sptrsv_bench_t
@@ -133,6 +133,7 @@ _PROTOS = {
     "sptrsv_memcpy_dtoh": (c_int, [c_void_p, c_void_p, c_size_t]),
     "sptrsv_memset": (c_int, [c_void_p, c_int, c_size_t]),
     "sptrsv_synchronize": (c_int, []),
+    "sptrsv_copy_device": (c_int, [c_void_p, c_void_p, c_size_t]),
     "sptrsv_timer_start": (c_int, [c_void_p]),
     "sptrsv_timer_stop": (c_int, [c_void_p, POINTER(c_float)]),
     "sptrsv_flush_cache": (c_int, [c_void_p]),

@@ -185,6 +185,9 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
         out->variant = 0;
     }
     if (out->cu_waves < 0) out->cu_waves = 0;
+    if (out->window != 0 && out->window != 1 && out->window != 4 && out->window != 16)
+        return fail(SPTRSV_E_INVALID_ARG, "window must be 0, 1, 4 or 16");
+    if (out->window == 16 && (K > 1 || out->wg != 256)) out->window = 4;
     if (out->lanes > kWave / K) out->lanes = kWave / K;
     if ((out->lanes & (out->lanes - 1)) != 0) return fail(SPTRSV_E_INVALID_ARG, "lanes must be a power of two");
     if (out->kernel == SPTRSV_KERNEL_ORDER && !has_order(info))
@@ -316,7 +319,7 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
         const int per_cu = c.cu_waves > 0 ? c.cu_waves : 4;
         A.persist_grid = (long long)device_cus() * std::max(1, per_cu / std::max(1, c.wg / kWave));
     }
-    int u = window_for(pv, (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW) ? 32 : c.lanes);
+    int u = c.window > 0 ? c.window : window_for(pv, (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW) ? 32 : c.lanes);
     hipEvent_t* ev = kernel_timer_pair(s);
     if (ev) SPT_HIP(hipEventRecord(ev[0], s));
     hipError_t e = launch_solve<P, 64>(kind, K, c.wg, u, A, s);
@@ -666,6 +669,24 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
                     best_t = t;
                     bc = c;
                 }
+            }
+        }
+    }
+    // Stage 3 (long rows): the best shape of each kernel with 16 entries per
+    // lane in flight instead of 4, which shortens the load chain of a long row.
+    if (status == SPTRSV_OK && K == 1 && pv->max_row > 64) {
+        for (int kern : {SPTRSV_KERNEL_SIMPLE, SPTRSV_KERNEL_ORDER, SPTRSV_KERNEL_MULTIROW}) {
+            auto it = std::find_if(ranked.begin(), ranked.end(),
+                                   [&](const auto& r) { return r.second.kernel == kern; });
+            if (it == ranked.end()) continue;
+            sptrsv_config_t c = (bc.kernel == kern) ? bc : it->second;
+            c.wg = 256;
+            c.window = 16;
+            const float t = time_cfg(c);
+            if (status != SPTRSV_OK) break;
+            if (t < best_t) {
+                best_t = t;
+                bc = c;
             }
         }
     }

@@ -34,8 +34,8 @@ int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, 
 int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best_ms);
 int reserve_jacobi(sp_mat_t* m, int k);
 void jacobi_host_tiles(const int* ia, int r_begin, int r_end, long long row_base, long long entry_base,
-                       std::vector<int>& out);
-int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>& tiles4, const std::vector<int>& tile_member,
+                       int window, std::vector<int>& out);
+int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vector<int>* tile_member,
                      int members, double delta);
 
 struct Task {
@@ -84,7 +84,7 @@ struct sptrsv_batch {
     std::vector<int> lanes;              // per matrix
     std::vector<int> avg;                // nnz/n per matrix
     std::vector<double> delta;           // per matrix: min_i |L_ii| - sum_j |L_ij|
-    std::vector<std::vector<int>> mtiles;  // per matrix: Jacobi row tiles (4 ints each, global)
+    std::vector<std::vector<int>> mtiles[2];  // per matrix: Jacobi row tiles of 1024 / 2048 entries
     std::vector<int> choice;             // MIXED per matrix: SPTRSV_KERNEL_SIMPLE or _JACOBI
     std::vector<float> t_sync;           // MIXED per matrix: tuned sync-free time alone (ms)
     sptrsv_config_t jac_cfg{};           // schedule of the Jacobi members
@@ -163,15 +163,17 @@ int build_tasks(sptrsv_batch_t* bt) {
 // Hands the tiles of the matrices whose choice is JACOBI to the Jacobi path.
 int install_jacobi(sptrsv_batch_t* bt) {
     const int nmat = (int)bt->choice.size();
-    std::vector<int> tiles, member;
+    std::vector<int> tiles[2], member[2];
     double delta = 1e300;
     bt->n_jacobi = 0;
     for (int m = 0; m < nmat; ++m) {
         if (bt->choice[m] != SPTRSV_KERNEL_JACOBI) continue;
         ++bt->n_jacobi;
         delta = std::min(delta, bt->delta[m]);
-        tiles.insert(tiles.end(), bt->mtiles[m].begin(), bt->mtiles[m].end());
-        member.insert(member.end(), bt->mtiles[m].size() / 4, m);
+        for (int w = 0; w < 2; ++w) {
+            tiles[w].insert(tiles[w].end(), bt->mtiles[w][m].begin(), bt->mtiles[w][m].end());
+            member[w].insert(member[w].end(), bt->mtiles[w][m].size() / 4, m);
+        }
     }
     SPT_TRY(jacobi_set_tiles(&bt->mat, tiles, member, nmat, bt->n_jacobi ? delta : 1.0));
     if (bt->n_jacobi) SPT_TRY(reserve_jacobi(&bt->mat, 1));
@@ -274,7 +276,8 @@ int sptrsv_batch_create(int nmat, const int* n, const int* const* ia, const int*
     bt->lanes.resize(nmat);
     bt->avg.resize(nmat);
     bt->delta.resize(nmat);
-    bt->mtiles.resize(nmat);
+    bt->mtiles[0].resize(nmat);
+    bt->mtiles[1].resize(nmat);
     bt->choice.assign(nmat, SPTRSV_KERNEL_SIMPLE);
     bt->t_sync.assign(nmat, 0.0f);
     long long entry_base = 0;
@@ -291,7 +294,9 @@ int sptrsv_batch_create(int nmat, const int* n, const int* const* ia, const int*
             dm = std::min(dm, std::fabs(a[m][ia[m][0] + re - 1]) - off);
         }
         bt->delta[m] = dm;
-        if (rows > 0) jacobi_host_tiles(ia[m], 0, n[m], bt->row_off[m], entry_base, bt->mtiles[m]);
+        if (rows > 0)
+            for (int w = 0; w < 2; ++w)
+                jacobi_host_tiles(ia[m], 0, n[m], bt->row_off[m], entry_base, 1024 << w, bt->mtiles[w][m]);
         entry_base += nz;
     }
     st = build_tasks(bt);

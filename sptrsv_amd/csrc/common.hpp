@@ -38,6 +38,15 @@ constexpr uint32_t kUnsolved32 = ~0u;
 // ---- library-private state hung off sp_mat_t::priv ------------------------
 struct Workspace;  // defined in solve.hip
 
+// Row tiles of the Jacobi passes: 4 ints per tile (first row, end row, first
+// entry, end entry) and, for a batch, the member of each tile (device).
+struct TileTable {
+    int* tiles = nullptr;
+    int* member = nullptr;   // null: one member
+    int count = 0;
+    int K = 0;               // right-hand-side width the tiles were cut for
+};
+
 struct MatPriv {
     int* counters = nullptr;     // [0] ticket, [1] watchdog word, [2..] spare (device)
     uint64_t* pub = nullptr;     // published words, n*k 64-bit (device)
@@ -58,18 +67,16 @@ struct MatPriv {
     double tol = 1e-10;          // certified relative accuracy
     double margin = 0.0;         // min_i |L_ii| - sum_j |L_ij|
     bool margin_known = false;
-    float* jac_r = nullptr;      // FP32 residual, n*k
+    float* jac_r = nullptr;      // FP32 row-scaled residual D^-1 r, n*k
+    float* jac_sval = nullptr;   // FP32 a_ij / a_ii (0 on the diagonal), nnz
     float* jac_a = nullptr;      // Jacobi iterates (ping-pong), n*k each
     float* jac_b = nullptr;
     size_t jac_len = 0;
     unsigned long long* jac_norms = nullptr;  // per-member max |r|, max |x| (double bits), replicated
     size_t jac_norms_len = 0;    // in 64-bit words
-    int* jac_tiles = nullptr;    // row tiles, 4 ints each: first row, end row, first entry, end entry
-    int jac_ntiles = 0;
-    int jac_tiles_K = 0;         // right-hand-side width the tiles were cut for
+    TileTable jac_tt[2];         // row tiles of 1024 / 2048 products (residual uses [0])
     bool jac_tiles_fixed = false;  // tiles installed by the caller (batch), never rebuilt
     int jac_members = 1;         // independent systems sharing the certificate (batch)
-    int* jac_tile_member = nullptr;  // member of each tile (device; null: one member)
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};
     int tuned_k[3] = {0, 0, 0};

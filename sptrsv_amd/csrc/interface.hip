@@ -78,11 +78,13 @@ static void free_mat(sp_mat_t* m) {
         if (pv->corr) (void)hipFree(pv->corr);
         if (pv->resid) (void)hipFree(pv->resid);
         if (pv->trace) (void)hipFree(pv->trace);
-        for (float* p : {pv->jac_r, pv->jac_a, pv->jac_b})
+        for (float* p : {pv->jac_r, pv->jac_a, pv->jac_b, pv->jac_sval})
             if (p) (void)hipFree(p);
         if (pv->jac_norms) (void)hipFree(pv->jac_norms);
-        if (pv->jac_tiles) (void)hipFree(pv->jac_tiles);
-        if (pv->jac_tile_member) (void)hipFree(pv->jac_tile_member);
+        for (TileTable& t : pv->jac_tt) {
+            if (t.tiles) (void)hipFree(t.tiles);
+            if (t.member) (void)hipFree(t.member);
+        }
         delete pv;
     }
     memset(m, 0, sizeof(*m));
@@ -168,6 +170,13 @@ static int upload(const int* ia, const int* ja, const V* val, int n, sp_mat_t* m
     return st;
 }
 
+
+// Plain streaming copy, 16 bytes per thread and step: the reference kernel
+// for scaling hardware traffic counters (reads n*16 bytes, writes n*16).
+static __global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 
 static __global__ void k_add_offset(int* p, size_t count, int off) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
@@ -378,6 +387,17 @@ int sptrsv_memset(void* dst, int value, size_t bytes) {
 }
 int sptrsv_synchronize(void) {
     SPT_HIP(hipDeviceSynchronize());
+    return SPTRSV_OK;
+}
+int sptrsv_copy_device(void* dst, const void* src, size_t bytes) {
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return fail(SPTRSV_E_INVALID_ARG, "copy needs 16-byte aligned pointers and size");
+    const size_t n = bytes / 16;
+    if (n == 0) return SPTRSV_OK;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3(g), dim3(256), 0, nullptr, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), n);
+    SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }
 

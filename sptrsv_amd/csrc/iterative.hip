@@ -12,9 +12,11 @@
 // HBM-bound passes.
 //
 // One refinement step (x in FP64):
-//   r = b - L x (FP64), r32 = (float) r, d_0 = r32 / diag       [k_jac_resid / k_jac_first]
-//   d_s = (r32 - N d_{s-1}) / diag, s = 1 .. m-1 (FP32)          [k_jac_sweep]
+//   r = b - L x (FP64), rs = (float)(D^-1 r) = d_0             [k_jac_resid / k_jac_first]
+//   d_s = rs - N' d_{s-1}, N' = D^-1 N in FP32, s = 1 .. m-1      [k_jac_sweep]
 //   x += d_{m-1} (FP64), fused into the last sweep
+// (the row scaling by D^-1 is applied once to the stored FP32 values, so a
+// sweep needs no diagonal)
 // Certificate: with delta = min_i(|L_ii| - sum_j |L_ij|) > 0, |L^-1|_inf <=
 // 1/delta, so |x - x*|_inf <= |b - L x|_inf / delta.  A final residual pass
 // takes max |r| and max |x| per independent system (one, or each member of a
@@ -24,13 +26,13 @@
 // Kernel layout (both passes): rows are cut into tiles by nnz windows -- tile
 // t holds the rows whose first entry lies in [t*W, (t+1)*W), W = 1024/K
 // entries for K right-hand sides -- so every tile streams about the same
-// number of entries.  A 256-thread workgroup stages its tile's row pointers,
-// diagonal and right-hand side in LDS, streams the entries with every lane
-// busy and all loads of a lane issued before their first use (phase 1: a
-// product per (entry, rhs) into LDS), then sums rows out of LDS with a lane
-// group per row sized to the tile's mean row length (phase 2).  A tile whose
-// products do not fit LDS (a very long row) sums straight from memory with a
-// wavefront per row.
+// number of entries.  A 256-thread workgroup stages its tile's row pointers
+// and right-hand side (and, for the residual, the diagonal) in LDS, streams
+// the entries with every lane busy and all loads of a lane issued before
+// their first use (phase 1: a product per (entry, rhs) into LDS), then sums
+// rows out of LDS with a lane group per row sized to the tile's mean row
+// length (phase 2).  A tile whose products do not fit LDS (a very long row)
+// sums straight from memory with a wavefront per row.
 #include <math.h>
 
 #include <algorithm>
@@ -40,14 +42,16 @@
 
 namespace sptrsv {
 
-constexpr int kTileCap = 2048;   // products staged in LDS per tile
 constexpr int kTileThreads = 256;
 constexpr int kNormReps = 16;    // replicas of each member's norm words (spread atomics)
 constexpr int kNormStride = 16;  // 64-bit words per replica (one 128-byte line)
 
-template <int K>
-__host__ __device__ constexpr int tile_window() {
-    return 1024 / K;
+// A tile spans an nnz window of TW / K entries for K right-hand sides (TW
+// products); LDS holds up to 5/4 of that, so a tile whose last row runs past
+// its window still fits unless that row is very long.
+template <int TW>
+__host__ __device__ constexpr int tile_cap() {
+    return TW + TW / 4;
 }
 
 struct TileArgs {
@@ -55,7 +59,7 @@ struct TileArgs {
     int k;
     const int* ia;
     const int* ja;
-    const float* a32;
+    const float* sval;        // a_ij / a_ii in FP32 (0 on the diagonal)
     const double* a64;
     const int4* tiles;        // per tile {first row, end row, first entry, end entry}
     const int* tile_member;   // member (independent system) of each tile; null: one member
@@ -88,20 +92,21 @@ __device__ __forceinline__ int group_lanes(int E, int rows) {
 }
 
 // Stages one tile in LDS with every independent load issued before the
-// first use: level 1 the tile bounds; level 2 the row pointers, the
-// per-row right-hand side (V: FP32 residual or FP64 b) and the entries;
-// level 3 the diagonals and the gathered iterate; then all LDS stores.
-// Products (a * v[ja], diagonal included) go to prod when the tile fits.
-template <int K, class V, class P>
+// first use: level 1 the tile record; level 2 the row pointers, the per-row
+// right-hand side (V) and the entries; level 3 the gathered iterate (and,
+// with DIAG, the diagonals); then all LDS stores.  Products a * v[ja]
+// (diagonal included) go to prod when the tile fits.
+template <int K, int TW, bool DIAG, class V, class P>
 struct TileStage {
-    static constexpr int W = tile_window<K>();
+    static constexpr int W = TW / K;                       // entries per window = max rows per tile
+    static constexpr int CAP = tile_cap<TW>();
     static constexpr int RU = (W + 1 + kTileThreads - 1) / kTileThreads;
-    static constexpr int SU = (W * K + kTileThreads - 1) / kTileThreads;
-    static constexpr int U = kTileCap / kTileThreads;
+    static constexpr int SU = (TW + kTileThreads - 1) / kTileThreads;
+    static constexpr int U = CAP / kTileThreads;
     int r0, rows, p0, E;
     bool lds;
 
-    __device__ bool run(const TileArgs& A, const V* rhs, const P* vals, const P* v, int* s_ia, float* s_dg,
+    __device__ bool run(const TileArgs& A, const V* rhs, const P* vals, const P* v, int* s_ia, double* s_dg,
                         V* s_rhs, P* prod) {
         const int tid = threadIdx.x;
         const int k = A.k;
@@ -111,7 +116,7 @@ struct TileStage {
         p0 = tl.z;
         E = tl.w - tl.z;
         if (rows <= 0) return false;
-        lds = E * K <= kTileCap;
+        lds = E * K <= CAP;
         // level 2
         int iav[RU];
 #pragma unroll
@@ -142,9 +147,11 @@ struct TileStage {
             }
         }
         // level 3
-        float dg[RU];
+        double dg[RU];
+        if (DIAG) {
 #pragma unroll
-        for (int u = 0; u < RU; ++u) dg[u] = (u * kTileThreads <= rows) ? A.a32[max(iav[u] - 1, 0)] : 1.0f;
+            for (int u = 0; u < RU; ++u) dg[u] = (u * kTileThreads <= rows) ? A.a64[max(iav[u] - 1, 0)] : 1.0;
+        }
         P xv[U];
         if (lds) {
 #pragma unroll
@@ -161,7 +168,7 @@ struct TileStage {
         for (int u = 0; u < RU; ++u) {
             const int t = tid + u * kTileThreads;
             if (t <= rows) s_ia[t] = iav[u] - p0;
-            if (t >= 1 && t <= rows) s_dg[t - 1] = dg[u];
+            if (DIAG && t >= 1 && t <= rows) s_dg[t - 1] = dg[u];
         }
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
@@ -179,20 +186,19 @@ struct TileStage {
     }
 };
 
-// One Jacobi sweep: d_new = (r32 - N d_old) / diag.  mode 0 stores d_new,
-// mode 1 sets x = d_new (first refinement step, x was 0), mode 2 adds
-// x += d_new.
-template <int K>
-__global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const float* __restrict__ r32,
+// One Jacobi sweep on the row-scaled system: d_new = rs - N' d_old, with
+// N' = D^-1 N stored as sval and rs = D^-1 r.  mode 0 stores d_new, mode 1
+// sets x = d_new (first refinement step, x was 0), mode 2 adds x += d_new.
+template <int K, int TW>
+__global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const float* __restrict__ rs,
                                                             const float* __restrict__ d_old,
                                                             float* __restrict__ d_new, double* x, int mode) {
-    constexpr int W = tile_window<K>();
-    __shared__ float prod[kTileCap];
-    __shared__ int s_ia[W + 1];
-    __shared__ float s_dg[W];
-    __shared__ float s_r[W * K];
-    TileStage<K, float, float> T;
-    if (!T.run(A, r32, A.a32, d_old, s_ia, s_dg, s_r, prod)) return;
+    using Stage = TileStage<K, TW, false, float, float>;
+    __shared__ float prod[Stage::CAP];
+    __shared__ int s_ia[Stage::W + 1];
+    __shared__ float s_r[TW];
+    Stage T;
+    if (!T.run(A, rs, A.sval, d_old, s_ia, nullptr, s_r, prod)) return;
     __syncthreads();
     const int tid = threadIdx.x;
     const int k = A.k;
@@ -207,19 +213,19 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const fl
         const bool live = lr < T.rows && rr < k;
         float s = 0.0f;
         if (live) {
-            const int rs = s_ia[lr], re = s_ia[lr + 1] - 1;  // diagonal excluded
+            const int rs0 = s_ia[lr], re = s_ia[lr + 1] - 1;  // diagonal excluded
             if (T.lds) {
-                for (int q = rs + e; q < re; q += G2) s += prod[q * K + rr];
+                for (int q = rs0 + e; q < re; q += G2) s += prod[q * K + rr];
             } else {
 #pragma unroll 4
-                for (int q = rs + e; q < re; q += G2)
-                    s = __fmaf_rn(A.a32[T.p0 + q], d_old[(size_t)A.ja[T.p0 + q] * k + rr], s);
+                for (int q = rs0 + e; q < re; q += G2)
+                    s = __fmaf_rn(A.sval[T.p0 + q], d_old[(size_t)A.ja[T.p0 + q] * k + rr], s);
             }
         }
         s = jac_group_sum(s, GL, K);
         if (live && e == 0) {
             const size_t i = (size_t)(T.r0 + lr) * k + rr;
-            const float d = (s_r[lr * K + rr] - s) * (1.0f / s_dg[lr]);
+            const float d = s_r[lr * K + rr] - s;
             if (mode == 0)
                 d_new[i] = d;
             else if (mode == 1)
@@ -234,21 +240,20 @@ __device__ __forceinline__ unsigned long long* norm_slot(const NormArgs& N, int 
     return N.norms + ((size_t)member * kNormReps + blockIdx.x % kNormReps) * kNormStride + word;
 }
 
-// r = b - L x in FP64 (diagonal included).  Writes r32 = (float) r when
-// r32_out is set, d0 = r32 / diag when d0 is set, and with N.norms folds
-// max |r| and max |x| per member.
+// r = b - L x in FP64 (diagonal included).  Writes rs = (float)(r / a_ii)
+// when rs_out is set, and with N.norms folds max |r| and max |x| per member.
 template <int K>
 __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const double* __restrict__ b,
                                                             const double* __restrict__ x,
-                                                            float* __restrict__ r32_out,
-                                                            float* __restrict__ d0, NormArgs N) {
-    constexpr int W = tile_window<K>();
-    __shared__ double prod[kTileCap];
-    __shared__ int s_ia[W + 1];
-    __shared__ float s_dg[W];
-    __shared__ double s_b[W * K];
+                                                            float* __restrict__ rs_out, NormArgs N) {
+    constexpr int TW = 1024;
+    using Stage = TileStage<K, TW, true, double, double>;
+    __shared__ double prod[Stage::CAP];
+    __shared__ int s_ia[Stage::W + 1];
+    __shared__ double s_dg[Stage::W];
+    __shared__ double s_b[TW];
     __shared__ unsigned long long s_nrm[2];
-    TileStage<K, double, double> T;
+    Stage T;
     if (!T.run(A, b, A.a64, x, s_ia, s_dg, s_b, prod)) return;
     const int tid = threadIdx.x;
     const int k = A.k;
@@ -264,12 +269,12 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
         const bool live = lr < T.rows && rr < k;
         double s = 0.0;
         if (live) {
-            const int rs = s_ia[lr], re = s_ia[lr + 1];
+            const int rs0 = s_ia[lr], re = s_ia[lr + 1];
             if (T.lds) {
-                for (int q = rs + e; q < re; q += G2) s += prod[q * K + rr];
+                for (int q = rs0 + e; q < re; q += G2) s += prod[q * K + rr];
             } else {
 #pragma unroll 4
-                for (int q = rs + e; q < re; q += G2)
+                for (int q = rs0 + e; q < re; q += G2)
                     s = __fma_rn(A.a64[T.p0 + q], x[(size_t)A.ja[T.p0 + q] * k + rr], s);
             }
         }
@@ -277,9 +282,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
         if (live && e == 0) {
             const size_t i = (size_t)(T.r0 + lr) * k + rr;
             const double r = s_b[lr * K + rr] - s;
-            const float rf = (float)r;
-            if (r32_out) r32_out[i] = rf;
-            if (d0) d0[i] = rf * (1.0f / s_dg[lr]);
+            if (rs_out) rs_out[i] = (float)(r / s_dg[lr]);
             if (N.norms) {
                 atomicMax(&s_nrm[0], (unsigned long long)__double_as_longlong(fabs(r)));
                 atomicMax(&s_nrm[1], (unsigned long long)__double_as_longlong(fabs(x[i])));
@@ -295,19 +298,26 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
     }
 }
 
-// First refinement step (x = 0): r32 = (float) b, d0 = r32 / diag.
+// First refinement step (x = 0): rs = (float)(b / a_ii).
 template <int K>
-__global__ __launch_bounds__(256) void k_jac_first(TileArgs A, const double* __restrict__ b, float* r32,
-                                                   float* d0) {
+__global__ __launch_bounds__(256) void k_jac_first(TileArgs A, const double* __restrict__ b, float* rs) {
     constexpr int lgK = kLog2<K>();
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long row = idx >> lgK;
     const int rr = (int)(idx & (K - 1));
     if (row >= A.n || rr >= A.k) return;
     const size_t i = (size_t)row * A.k + rr;
-    const float rf = (float)b[i];
-    r32[i] = rf;
-    d0[i] = rf * (1.0f / A.a32[A.ia[row + 1] - 1]);
+    rs[i] = (float)(b[i] / A.a64[A.ia[row + 1] - 1]);
+}
+
+// sval = a_ij / a_ii in FP32 for j < i, 0 on the diagonal (one row per thread).
+__global__ void k_jac_scale(int n, const int* __restrict__ ia, const double* __restrict__ a, float* sval) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int rs = ia[i], re = ia[i + 1];
+        const double dg = a[re - 1];
+        for (int p = rs; p < re - 1; ++p) sval[p] = (float)(a[p] / dg);
+        sval[re - 1] = 0.0f;
+    }
 }
 
 // Row tiles by nnz windows: tile t = {first row whose first entry is >=
@@ -435,21 +445,29 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
         SPT_HIP(hipMalloc(&pv->jac_norms, nw * sizeof(unsigned long long)));
         pv->jac_norms_len = nw;
     }
-    const int K = pow2_at_least(k);
-    if (pv->jac_tiles_K != K) {
-        if (pv->jac_tiles_fixed) return fail(SPTRSV_E_CAPACITY, "batched JACOBI solves take one right-hand side");
-        if (pv->jac_tiles) SPT_HIP(hipFree(pv->jac_tiles));
-        pv->jac_tiles = nullptr;
-        pv->jac_tiles_K = 0;
-        const int window = 1024 / K;
-        const int ntiles = (int)std::max<long long>(1, ((long long)m->nnz + window - 1) / window);
-        SPT_HIP(hipMalloc(&pv->jac_tiles, 4 * sizeof(int) * (size_t)ntiles));
-        hipLaunchKernelGGL(k_jac_tiles, dim3(ceil_div(ntiles, 256)), dim3(256), 0, nullptr, m->ia, m->nr,
-                           window, ntiles, reinterpret_cast<int4*>(pv->jac_tiles));
+    if (!pv->jac_sval) {
+        SPT_HIP(hipMalloc(&pv->jac_sval, sizeof(float) * (size_t)std::max(1, m->nnz)));
+        const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(4096, ((long long)m->nr + 255) / 256));
+        hipLaunchKernelGGL(k_jac_scale, dim3(g), dim3(256), 0, nullptr, m->nr, m->ia, m->a, pv->jac_sval);
         SPT_HIP(hipGetLastError());
         SPT_HIP(hipDeviceSynchronize());
-        pv->jac_ntiles = ntiles;
-        pv->jac_tiles_K = K;
+    }
+    const int K = pow2_at_least(k);
+    for (int w = 0; w < 2; ++w) {
+        TileTable& t = pv->jac_tt[w];
+        if (t.K == K) continue;
+        if (pv->jac_tiles_fixed) return fail(SPTRSV_E_CAPACITY, "batched JACOBI solves take one right-hand side");
+        if (t.tiles) SPT_HIP(hipFree(t.tiles));
+        t = TileTable{};
+        const int window = (1024 << w) / K;
+        const int ntiles = (int)std::max<long long>(1, ((long long)m->nnz + window - 1) / window);
+        SPT_HIP(hipMalloc(&t.tiles, 4 * sizeof(int) * (size_t)ntiles));
+        hipLaunchKernelGGL(k_jac_tiles, dim3(ceil_div(ntiles, 256)), dim3(256), 0, nullptr, m->ia, m->nr,
+                           window, ntiles, reinterpret_cast<int4*>(t.tiles));
+        SPT_HIP(hipGetLastError());
+        SPT_HIP(hipDeviceSynchronize());
+        t.count = ntiles;
+        t.K = K;
     }
     return SPTRSV_OK;
 }
@@ -459,11 +477,11 @@ int reserve_jacobi(sp_mat_t* m, int k) { return ensure_jacobi_ws(m, k); }
 // Host-side tiles of rows [r_begin, r_end) of a CSR whose row pointers are
 // ia (relative: ia[r] - ia[r_begin] is the entry offset inside the range),
 // appended as {first row, end row, first entry, end entry} with rows and
-// entries shifted by row_base / entry_base.  Same windows as k_jac_tiles,
-// restarted at the range's first entry, so tiles never straddle ranges.
+// entries shifted by row_base / entry_base.  Same windows (of `window`
+// entries) as k_jac_tiles, restarted at the range's first entry, so tiles
+// never straddle ranges.
 void jacobi_host_tiles(const int* ia, int r_begin, int r_end, long long row_base, long long entry_base,
-                       std::vector<int>& out) {
-    const int window = 1024;
+                       int window, std::vector<int>& out) {
     const long long e0 = ia[r_begin];
     const long long nz = (long long)ia[r_end] - e0;
     int r = r_begin;
@@ -478,30 +496,31 @@ void jacobi_host_tiles(const int* ia, int r_begin, int r_end, long long row_base
     }
 }
 
-// Installs a tile table built on the host (one right-hand side) covering the
-// rows the JACOBI path should solve, with the member of each tile; members
-// without tiles are left to other solvers.  delta is the dominance margin of
-// the covered rows.
-int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>& tiles4, const std::vector<int>& tile_member,
+// Installs tile tables built on the host (one right-hand side; [0] windows of
+// 1024 entries, [1] of 2048) covering the rows the JACOBI path should solve,
+// with the member of each tile; members without tiles are left to other
+// solvers.  delta is the dominance margin of the covered rows.
+int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vector<int>* tile_member,
                      int members, double delta) {
     MatPriv* pv = priv_of(m);
-    if (pv->jac_tiles) SPT_HIP(hipFree(pv->jac_tiles));
-    if (pv->jac_tile_member) SPT_HIP(hipFree(pv->jac_tile_member));
-    pv->jac_tiles = nullptr;
-    pv->jac_tile_member = nullptr;
-    pv->jac_ntiles = 0;
-    pv->jac_tiles_K = 1;
     pv->jac_tiles_fixed = true;
     pv->jac_members = std::max(1, members);
     pv->margin = delta;
     pv->margin_known = true;
-    const size_t nt = tile_member.size();
-    if (nt == 0) return SPTRSV_OK;
-    SPT_HIP(hipMalloc(&pv->jac_tiles, sizeof(int) * tiles4.size()));
-    SPT_HIP(hipMemcpy(pv->jac_tiles, tiles4.data(), sizeof(int) * tiles4.size(), hipMemcpyHostToDevice));
-    SPT_HIP(hipMalloc(&pv->jac_tile_member, sizeof(int) * nt));
-    SPT_HIP(hipMemcpy(pv->jac_tile_member, tile_member.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
-    pv->jac_ntiles = (int)nt;
+    for (int w = 0; w < 2; ++w) {
+        TileTable& t = pv->jac_tt[w];
+        if (t.tiles) SPT_HIP(hipFree(t.tiles));
+        if (t.member) SPT_HIP(hipFree(t.member));
+        t = TileTable{};
+        t.K = 1;
+        const size_t nt = tile_member[w].size();
+        if (nt == 0) continue;
+        SPT_HIP(hipMalloc(&t.tiles, sizeof(int) * tiles4[w].size()));
+        SPT_HIP(hipMemcpy(t.tiles, tiles4[w].data(), sizeof(int) * tiles4[w].size(), hipMemcpyHostToDevice));
+        SPT_HIP(hipMalloc(&t.member, sizeof(int) * nt));
+        SPT_HIP(hipMemcpy(t.member, tile_member[w].data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+        t.count = (int)nt;
+    }
     return SPTRSV_OK;
 }
 
@@ -511,39 +530,50 @@ template <int K>
 static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, double* x,
                            hipStream_t s) {
     MatPriv* pv = priv_of(m);
-    TileArgs A{m->nr, k, m->ia, m->ja, m->a32, m->a, reinterpret_cast<const int4*>(pv->jac_tiles),
-               pv->jac_tile_member};
+    const TileTable& t0 = pv->jac_tt[0];
+    const int wide = (c.lanes == 2 && pv->jac_tt[1].count > 0) ? 1 : 0;  // sweep tiles: 1024 or 2048
+    const TileTable& ts = pv->jac_tt[wide];
+    TileArgs A{m->nr, k, m->ia, m->ja, pv->jac_sval, m->a, reinterpret_cast<const int4*>(t0.tiles), t0.member};
+    TileArgs AS = A;
+    AS.tiles = reinterpret_cast<const int4*>(ts.tiles);
+    AS.tile_member = ts.member;
     const NormArgs none{nullptr, 1};
     const NormArgs with{pv->jac_norms, std::max(1, pv->jac_members)};
-    if (pv->jac_ntiles == 0) return SPTRSV_OK;
-    const unsigned tiles = (unsigned)pv->jac_ntiles;
+    if (t0.count == 0) return SPTRSV_OK;
+    const unsigned tiles = (unsigned)t0.count;
     const unsigned first_grid = ceil_div((long long)m->nr * K, 256);
     const int sweeps = std::max(2, c.sweeps);
     const int outer = std::max(1, c.outer);
+    float* rs = pv->jac_r;  // D^-1 r of the current step; also the first iterate
     for (int o = 0; o < outer; ++o) {
         if (o == 0)
-            hipLaunchKernelGGL((k_jac_first<K>), dim3(first_grid), dim3(256), 0, s, A, b, pv->jac_r, pv->jac_a);
+            hipLaunchKernelGGL((k_jac_first<K>), dim3(first_grid), dim3(256), 0, s, A, b, rs);
         else
             hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b,
-                               (const double*)x, pv->jac_r, pv->jac_a, none);
-        float* cur = pv->jac_a;
-        float* nxt = pv->jac_b;
+                               (const double*)x, rs, none);
+        const float* cur = rs;
+        float* nxt = pv->jac_a;
         for (int sw = 1; sw < sweeps; ++sw) {
             const bool last = sw == sweeps - 1;
             const int mode = last ? (o == 0 ? 1 : 2) : 0;
             hipEvent_t* ev = kernel_timer_pair(s);
             if (ev) SPT_HIP(hipEventRecord(ev[0], s));
-            hipLaunchKernelGGL((k_jac_sweep<K>), dim3(tiles), dim3(kTileThreads), 0, s, A,
-                               (const float*)pv->jac_r, (const float*)cur, nxt, x, mode);
+            if (wide)
+                hipLaunchKernelGGL((k_jac_sweep<K, 2048>), dim3(ts.count), dim3(kTileThreads), 0, s, AS,
+                                   (const float*)rs, cur, nxt, x, mode);
+            else
+                hipLaunchKernelGGL((k_jac_sweep<K, 1024>), dim3(ts.count), dim3(kTileThreads), 0, s, AS,
+                                   (const float*)rs, cur, nxt, x, mode);
             if (ev) SPT_HIP(hipEventRecord(ev[1], s));
-            std::swap(cur, nxt);
+            cur = nxt;
+            nxt = (nxt == pv->jac_a) ? pv->jac_b : pv->jac_a;
         }
     }
     // certificate: final FP64 residual with per-member max-norms
     const size_t nw = (size_t)with.members * kNormReps * kNormStride;
     hipLaunchKernelGGL(k_jac_zero_norms, dim3(ceil_div((long long)nw, 256)), dim3(256), 0, s, pv->jac_norms, nw);
     hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b, (const double*)x,
-                       (float*)nullptr, (float*)nullptr, with);
+                       (float*)nullptr, with);
     hipLaunchKernelGGL(k_jac_certify, dim3(ceil_div(with.members, 256)), dim3(256), 0, s, pv->jac_norms,
                        with.members, pv->margin, pv->tol, pv->counters + 3);
     SPT_HIP(hipGetLastError());
@@ -589,7 +619,7 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
     int st = SPTRSV_OK;
     int prev_outer = 16;
     for (int sweeps : {4, 6, 8, 12, 16, 24}) {
-        sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, 1, 256, 1, 0, 0, sweeps, 1};
+        sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, 1, 256, 1, 0, 0, sweeps, 1, 0};
         // the smallest number of refinement steps that certifies tol/8
         bool found = false;
         pv->tol = tol / 8;
@@ -609,16 +639,20 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
         if (st != SPTRSV_OK) break;
         if (!found) continue;
         prev_outer = c.outer;  // more sweeps per step never need more steps
-        SPT_HIP(hipEventRecord(e0, nullptr));
-        for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
-        SPT_HIP(hipEventRecord(e1, nullptr));
-        SPT_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= std::max(reps, 1);
-        if (ms < bt) {
-            bt = ms;
-            bc = c;
+        for (int width : {1, 2}) {  // sweep tiles of 1024 or 2048 products
+            c.lanes = width;
+            st = jacobi_solve(m, c, k, b, x, nullptr);
+            SPT_HIP(hipEventRecord(e0, nullptr));
+            for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
+            SPT_HIP(hipEventRecord(e1, nullptr));
+            SPT_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= std::max(reps, 1);
+            if (st == SPTRSV_OK && ms < bt) {
+                bt = ms;
+                bc = c;
+            }
         }
         if (st != SPTRSV_OK) break;
     }

@@ -786,15 +786,16 @@ hipError_t launch_k(int kind, const SolveArgs& A, hipStream_t s) {
 }
 
 // Launch one solve pass.  K: right-hand-side lanes (1..64, power of two);
-// with K == 1 the workgroup may be 256 or 1024 threads and the entry window
-// 1 or 4; with K > 1 both are fixed at 256 and 1.
+// with K == 1 the workgroup may be 64, 256 or 1024 threads and the entry
+// window 1 or 4 (16 with 256 threads); with K > 1 both are fixed at 256 and 1.
 template <class P, int MAXK>
 hipError_t launch_solve(int kind, int K, int wg, int u, const SolveArgs& A, hipStream_t s) {
     if (K == 1) {
         if (wg == 1024)
-            return (u == 4) ? launch_k<P, 1, 1024, 4>(kind, A, s) : launch_k<P, 1, 1024, 1>(kind, A, s);
+            return (u >= 4) ? launch_k<P, 1, 1024, 4>(kind, A, s) : launch_k<P, 1, 1024, 1>(kind, A, s);
         if (wg == 64)
-            return (u == 4) ? launch_k<P, 1, 64, 4>(kind, A, s) : launch_k<P, 1, 64, 1>(kind, A, s);
+            return (u >= 4) ? launch_k<P, 1, 64, 4>(kind, A, s) : launch_k<P, 1, 64, 1>(kind, A, s);
+        if (u == 16) return launch_k<P, 1, 256, 16>(kind, A, s);
         return (u == 4) ? launch_k<P, 1, 256, 4>(kind, A, s) : launch_k<P, 1, 256, 1>(kind, A, s);
     }
     if constexpr (MAXK > 1) {

@@ -198,15 +198,17 @@ def csr_L_solve_mixed(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, ref
                                     mat.nr, k, refine_steps, stream), "csr_L_solve_mixed")
 
 
-def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0, sweeps=0, outer=0) -> Config:
-    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer)
+def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0, sweeps=0, outer=0,
+         window=0) -> Config:
+    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window)
 
 
 def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
           kernel: int = KERNEL_AUTO, lanes: int = 0, wg: int = 0, fused: int = 1, variant: int = 0,
-          cu_waves: int = 0, sweeps: int = 0, outer: int = 0, stream=None, cfg: Config | None = None) -> None:
+          cu_waves: int = 0, sweeps: int = 0, outer: int = 0, window: int = 0, stream=None,
+          cfg: Config | None = None) -> None:
     """cfg, when given, is used as is (e.g. the result of autotune)."""
-    cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer)
+    cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window)
     check(L.lib().sptrsv_solve(byref(mat), byref(info) if info is not None else None, precision, byref(cfg),
                                b.ptr, x.ptr, mat.nr, k, refine_steps, stream), "sptrsv_solve")
 
@@ -393,6 +395,10 @@ def device_name(device: int = 0) -> str:
     buf = ctypes.create_string_buffer(256)
     check(L.lib().sptrsv_device_name(device, buf, 256), "device_name")
     return buf.value.decode()
+
+
+def copy_device(dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
+    check(L.lib().sptrsv_copy_device(dst.ptr, src.ptr, nbytes), "sptrsv_copy_device")
 
 
 def synchronize() -> None:

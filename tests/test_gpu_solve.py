@@ -44,12 +44,13 @@ class Dev:
         self.info = I.multirow_analysis_base_GPU(self.mat, I.ALL) if analyse else None
 
     def solve(self, b, precision=I.PREC_F64, kernel=I.KERNEL_SIMPLE, k=1, lanes=0, wg=0, fused=1, refine=1,
-              variant=0, cu_waves=0, sweeps=0, outer=0, expect=0):
+              variant=0, cu_waves=0, sweeps=0, outer=0, expect=0, window=0):
         dt = np.float32 if precision == I.PREC_F32 else np.float64
         db = I.DeviceArray.from_numpy(np.asarray(b, dt))
         dx = I.DeviceArray(self.n * k, dt)
         I.solve(self.mat, self.info, precision, db, dx, k=k, refine_steps=refine, kernel=kernel, lanes=lanes,
-                wg=wg, fused=fused, variant=variant, cu_waves=cu_waves, sweeps=sweeps, outer=outer)
+                wg=wg, fused=fused, variant=variant, cu_waves=cu_waves, sweeps=sweeps, outer=outer,
+                window=window)
         assert I.check_watchdog(self.mat) == expect, L.last_error()
         x = dx.numpy()
         db.free()
@@ -239,6 +240,21 @@ def test_fp64_fp32_mixed_configs(device, cfg, param):
                 xm = d.solve(s.b, precision=I.PREC_MIXED, kernel=kern, fused=fused, lanes=lanes,
                              variant=variant)
                 assert relerr(xm, ref) <= 1e-10, (name, fused, variant)
+    d.close()
+
+
+def test_long_row_window(device):
+    """16 entries per lane in flight (power-law rows up to 2000 entries)."""
+    s = I.generate(3, 100000)
+    ref = O.fwd_f64(s.ia, s.ja, s.a, s.b)
+    d = Dev(s.ia, s.ja, s.a)
+    for name, kern in KERNELS.items():
+        for variant in (0, 1, 2):
+            x = d.solve(s.b, kernel=kern, lanes=32 if variant else 0, wg=256, variant=variant, window=16)
+            assert relerr(x, ref) <= 1e-12, (name, variant)
+            xm = d.solve(s.b, precision=I.PREC_MIXED, kernel=kern, lanes=32 if variant else 0, wg=256,
+                         variant=variant, window=16)
+            assert relerr(xm, ref) <= 1e-10, (name, variant)
     d.close()
 
 
