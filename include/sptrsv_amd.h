@@ -195,7 +195,8 @@ typedef struct {
     int sweeps;   /* JACOBI: FP32 sweeps per refinement step */
     int outer;    /* JACOBI: refinement steps */
     int window;   /* sync-free: entries per lane kept in flight, 1, 4 or 16
-                     (16: one right-hand side, wg 256); 0 = by longest row */
+                     (16: one right-hand side, wg 256); 0 = by longest row.
+                     JACOBI: entries of a row one lane sums (0 = 8) */
 } sptrsv_config_t;
 
 /* Relative accuracy the JACOBI path certifies (default 1e-10). */
@@ -212,6 +213,14 @@ int sptrsv_solve(sp_mat_t* mat, dfr_analysis_info_t* info, int precision,
                  const sptrsv_config_t* cfg, const void* b, void* x, int n,
                  int k, int refine_steps, hipStream_t stream);
 
+/* sptrsv_solve captured once into a hipGraph per (precision, b, x, n, k,
+ * refine_steps, resolved configuration) and replayed on `stream`: one
+ * launch per solve instead of one per pass (a Jacobi solve has ~30).  The
+ * first call for a key runs one eager solve, then captures; later calls
+ * only launch.  The matrix owns the graphs (freed with it). */
+int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision,
+                       const sptrsv_config_t* cfg, const void* b, void* x, int n, int k,
+                       int refine_steps, hipStream_t stream);
 /* Mixed-precision solve (FP32 bulk + FP64 residual and update), k RHS. */
 int csr_L_solve_mixed(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b,
                       double* x, int n, int k, int refine_steps, hipStream_t stream);

@@ -89,6 +89,8 @@ _PROTOS = {
     "csr_L_solve_multirow_f32": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int, c_void_p]),
     "sptrsv_solve": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_int, POINTER(Config), c_void_p,
                              c_void_p, c_int, c_int, c_int, c_void_p]),
+    "sptrsv_solve_graph": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_int, POINTER(Config), c_void_p,
+                                   c_void_p, c_int, c_int, c_int, c_void_p]),
     "csr_L_solve_mixed": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int, c_int,
                                   c_int, c_void_p]),
     "sptrsv_reserve": (c_int, [POINTER(SpMat), c_int]),

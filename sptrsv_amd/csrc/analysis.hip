@@ -160,7 +160,7 @@ static int exclusive_scan(const int* in, int* out, long long n, hipStream_t s) {
                            sums);
         if (hipGetLastError() != hipSuccess) st = fail(SPTRSV_E_HIP, "scan apply launch");
     }
-    hipStreamSynchronize(s);
+    if (hipStreamSynchronize(s) != hipSuccess && st == SPTRSV_OK) st = fail(SPTRSV_E_HIP, "scan sync");
     (void)hipFree(sums);
     return st;
 }
@@ -256,7 +256,7 @@ static int radix_sort_pairs(int* keys, int* vals, int* tk, int* tv, int n, int m
             hipMemcpyAsync(vals, vi, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, s) != hipSuccess)
             st = fail(SPTRSV_E_HIP, "radix copy back");
     }
-    hipStreamSynchronize(s);
+    if (hipStreamSynchronize(s) != hipSuccess && st == SPTRSV_OK) st = fail(SPTRSV_E_HIP, "radix sort sync");
     (void)hipFree(hist);
     return st;
 }
@@ -430,7 +430,7 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
     (void)hipFree(d_max);
     if (h_max[1]) {
         int zero = 0;
-        hipMemcpy(pv->counters + 1, &zero, sizeof(int), hipMemcpyHostToDevice);
+        (void)hipMemcpy(pv->counters + 1, &zero, sizeof(int), hipMemcpyHostToDevice);
         return fail(SPTRSV_E_WATCHDOG, "level analysis: a dependency wait exceeded its bound");
     }
     const int nlevs = h_max[0];
@@ -498,7 +498,7 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
         }
         if (warp_off) (void)hipFree(warp_off);
     }
-    hipFree(bucket_cnt);
+    (void)hipFree(bucket_cnt);
     if (st != SPTRSV_OK) return st;
     SPT_HIP(hipStreamSynchronize(s));
     return SPTRSV_OK;

@@ -182,12 +182,16 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     if (out->wg != 64 && out->wg != 256 && out->wg != 1024) out->wg = 256;
     if (K > 1) {
         out->wg = 256;
-        out->variant = 0;
+        if (out->kernel != SPTRSV_KERNEL_JACOBI) out->variant = 0;  // no lean path with several RHS
     }
     if (out->cu_waves < 0) out->cu_waves = 0;
-    if (out->window != 0 && out->window != 1 && out->window != 4 && out->window != 16)
-        return fail(SPTRSV_E_INVALID_ARG, "window must be 0, 1, 4 or 16");
-    if (out->window == 16 && (K > 1 || out->wg != 256)) out->window = 4;
+    if (out->kernel == SPTRSV_KERNEL_JACOBI) {
+        if (out->window < 0 || out->window > 64) return fail(SPTRSV_E_INVALID_ARG, "window must be 0..64");
+    } else {
+        if (out->window != 0 && out->window != 1 && out->window != 4 && out->window != 16)
+            return fail(SPTRSV_E_INVALID_ARG, "window must be 0, 1, 4 or 16");
+        if (out->window == 16 && (K > 1 || out->wg != 256)) out->window = 4;
+    }
     if (out->lanes > kWave / K) out->lanes = kWave / K;
     if ((out->lanes & (out->lanes - 1)) != 0) return fail(SPTRSV_E_INVALID_ARG, "lanes must be a power of two");
     if (out->kernel == SPTRSV_KERNEL_ORDER && !has_order(info))
@@ -233,7 +237,7 @@ static int reserve(sp_mat_t* m, int k, bool need_pub, bool need_two_pass) {
 constexpr unsigned kLdsPerCu = 160u * 1024u;
 constexpr unsigned kStaticLdsSlack = 8u * 1024u + 512u;  // largest static LDS of a solve kernel
 
-static int device_cus() {
+int device_cus() {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
@@ -260,7 +264,10 @@ static std::vector<hipEvent_t> g_kt_pool;   // pairs
 static size_t g_kt_used = 0;
 
 // Next free (start, stop) event pair when timing is enabled, else nullptr.
+static thread_local bool g_capturing = false;  // no timer events inside a capture
+
 hipEvent_t* kernel_timer_pair(hipStream_t) {
+    if (g_capturing) return nullptr;
     std::lock_guard<std::mutex> g(g_kt_mu);
     if (!g_kt_on) return nullptr;
     if (g_kt_used + 2 > g_kt_pool.size()) {
@@ -426,6 +433,44 @@ extern "C" {
 int sptrsv_solve(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, const sptrsv_config_t* cfg,
                  const void* b, void* x, int n, int k, int refine_steps, hipStream_t stream) {
     return solve_checked(mat, info, precision, cfg, b, x, n, k, refine_steps, stream);
+}
+
+int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, const sptrsv_config_t* cfg,
+                       const void* b, void* x, int n, int k, int refine_steps, hipStream_t stream) {
+    SPT_TRY(check_solve_args(mat, precision, b, x, n, k, refine_steps));
+    if (n == 0) return SPTRSV_OK;
+    sptrsv_config_t c;
+    SPT_TRY(resolve_config(mat, info, precision, k, cfg, &c));
+    MatPriv* pv = priv_of(mat);
+    const std::array<long long, 15> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
+                                        refine_steps, c.kernel, c.lanes, c.wg, c.fused, c.variant, c.cu_waves,
+                                        c.sweeps, c.outer, c.window};
+    auto it = pv->graphs.find(key);
+    if (it == pv->graphs.end()) {
+        // one eager solve sets up every workspace and table (no allocation
+        // may happen inside the capture)
+        SPT_TRY(solve_impl(mat, info, precision, c, b, x, k, refine_steps, stream));
+        SPT_HIP(hipStreamSynchronize(stream));
+        if (!pv->cap_stream) SPT_HIP(hipStreamCreate(&pv->cap_stream));
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        SPT_HIP(hipStreamBeginCapture(pv->cap_stream, hipStreamCaptureModeThreadLocal));
+        g_capturing = true;
+        const int st = solve_impl(mat, info, precision, c, b, x, k, refine_steps, pv->cap_stream);
+        g_capturing = false;
+        hipError_t e = hipStreamEndCapture(pv->cap_stream, &g);
+        if (st != SPTRSV_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return hip_fail(e, "end capture", __FILE__, __LINE__);
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "graph instantiate", __FILE__, __LINE__);
+        it = pv->graphs.emplace(key, ge).first;
+    }
+    SPT_HIP(hipGraphLaunch(it->second, stream));
+    return SPTRSV_OK;
 }
 
 int sptrsv_default_config(const sp_mat_t* mat, const dfr_analysis_info_t* info, int precision, int k,

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
+#include <map>
 #include <string>
 
 #include "sptrsv_amd.h"
@@ -45,6 +47,16 @@ struct TileTable {
     int* member = nullptr;   // null: one member
     int count = 0;
     int K = 0;               // right-hand-side width the tiles were cut for
+    // entries of each tile sorted by column (built on request): column,
+    // scaled value, position inside the tile
+    int* sja = nullptr;
+    float* ssv = nullptr;
+    unsigned short* sslot = nullptr;
+    // 16-bit columns (r0 - column) in storage and sorted order; c16_state
+    // 0 not built, 1 built, -1 some tile's columns do not fit 16 bits
+    short* c16 = nullptr;
+    short* sc16 = nullptr;
+    int c16_state = 0;
 };
 
 struct MatPriv {
@@ -77,6 +89,10 @@ struct MatPriv {
     TileTable jac_tt[2];         // row tiles of 1024 / 2048 products (residual uses [0])
     bool jac_tiles_fixed = false;  // tiles installed by the caller (batch), never rebuilt
     int jac_members = 1;         // independent systems sharing the certificate (batch)
+    // captured solves (sptrsv_solve_graph): key = precision, b, x, n, k,
+    // refinement steps and the resolved configuration
+    std::map<std::array<long long, 15>, hipGraphExec_t> graphs;
+    hipStream_t cap_stream = nullptr;
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};
     int tuned_k[3] = {0, 0, 0};

@@ -81,9 +81,16 @@ static void free_mat(sp_mat_t* m) {
         for (float* p : {pv->jac_r, pv->jac_a, pv->jac_b, pv->jac_sval})
             if (p) (void)hipFree(p);
         if (pv->jac_norms) (void)hipFree(pv->jac_norms);
+        for (auto& g : pv->graphs) (void)hipGraphExecDestroy(g.second);
+        if (pv->cap_stream) (void)hipStreamDestroy(pv->cap_stream);
         for (TileTable& t : pv->jac_tt) {
             if (t.tiles) (void)hipFree(t.tiles);
             if (t.member) (void)hipFree(t.member);
+            if (t.sja) (void)hipFree(t.sja);
+            if (t.ssv) (void)hipFree(t.ssv);
+            if (t.sslot) (void)hipFree(t.sslot);
+            if (t.c16) (void)hipFree(t.c16);
+            if (t.sc16) (void)hipFree(t.sc16);
         }
         delete pv;
     }

@@ -63,6 +63,15 @@ struct TileArgs {
     const double* a64;
     const int4* tiles;        // per tile {first row, end row, first entry, end entry}
     const int* tile_member;   // member (independent system) of each tile; null: one member
+    // column-sorted tiles (sweep only): each tile's entries ordered by column,
+    // with the entry's position inside its tile; null: storage order
+    const int* sja;
+    const float* ssv;
+    const unsigned short* sslot;
+    // 16-bit columns, as r0 - column with r0 the tile's first row (storage
+    // and sorted order); null: 32-bit columns
+    const short* c16;
+    const short* sc16;
 };
 
 struct NormArgs {
@@ -81,14 +90,25 @@ __device__ __forceinline__ T jac_group_sum(T v, int G, int K) {
     return v;
 }
 
-// Lane-group shape for phase 2: G2 lanes per row and right-hand side,
-// the smallest power of two covering the mean row length (at most 64 / K).
+// Lane-group shape for phase 2: G2 lanes per row and right-hand side, the
+// smallest power of two for which a lane sums at most `epl` entries of a
+// row of the tile's mean length (at most 64 / K lanes).
 template <int K>
-__device__ __forceinline__ int group_lanes(int E, int rows) {
+__device__ __forceinline__ int group_lanes(int E, int rows, int epl) {
     const int avg = (E + rows - 1) / rows;
     int g = 1;
-    while (g < avg && g * K < kWave) g <<= 1;
+    while (g * epl < avg && g * K < kWave) g <<= 1;
     return g;
+}
+
+// Tile of workgroup b.  Workgroups are dispatched to the 8 XCDs round-robin
+// (b mod 8), each with its own L2; this bijection hands XCD x one contiguous
+// eighth of the tiles, so neighbouring tiles -- which gather the same
+// stretch of the iterate -- share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+    const int q = nblocks >> 3, r = nblocks & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + min(x, r) + i;
 }
 
 // Stages one tile in LDS with every independent load issued before the
@@ -96,7 +116,7 @@ __device__ __forceinline__ int group_lanes(int E, int rows) {
 // right-hand side (V) and the entries; level 3 the gathered iterate (and,
 // with DIAG, the diagonals); then all LDS stores.  Products a * v[ja]
 // (diagonal included) go to prod when the tile fits.
-template <int K, int TW, bool DIAG, class V, class P>
+template <int K, int TW, bool DIAG, class V, class P, bool SORTED = false, bool COL16 = false>
 struct TileStage {
     static constexpr int W = TW / K;                       // entries per window = max rows per tile
     static constexpr int CAP = tile_cap<TW>();
@@ -110,7 +130,7 @@ struct TileStage {
                         V* s_rhs, P* prod) {
         const int tid = threadIdx.x;
         const int k = A.k;
-        const int4 tl = A.tiles[blockIdx.x];
+        const int4 tl = A.tiles[xcd_tile(blockIdx.x, gridDim.x)];
         r0 = tl.x;
         rows = tl.y - tl.x;
         p0 = tl.z;
@@ -132,17 +152,24 @@ struct TileStage {
                 rv[u] = rhs[(size_t)(r0 + lr) * k + rr];
             }
         }
-        int jv[U];
+        int jv[U], slot[U];
         P av[U];
         if (lds) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 jv[u] = 0;
                 av[u] = P(0);
+                slot[u] = 0;
                 if (u * kTileThreads < E * K) {
                     const int e = min((tid + u * kTileThreads) / K, E - 1);
-                    jv[u] = A.ja[p0 + e];
-                    av[u] = vals[p0 + e];
+                    if (SORTED) {
+                        jv[u] = COL16 ? r0 - (int)A.sc16[p0 + e] : A.sja[p0 + e];
+                        av[u] = (P)A.ssv[p0 + e];
+                        slot[u] = A.sslot[p0 + e];
+                    } else {
+                        jv[u] = COL16 ? r0 - (int)A.c16[p0 + e] : A.ja[p0 + e];
+                        av[u] = vals[p0 + e];
+                    }
                 }
             }
         }
@@ -179,7 +206,7 @@ struct TileStage {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int t = tid + u * kTileThreads;
-                if (t < E * K) prod[t] = av[u] * xv[u];
+                if (t < E * K) prod[SORTED ? slot[u] * K + (t & (K - 1)) : t] = av[u] * xv[u];
             }
         }
         return true;
@@ -189,11 +216,12 @@ struct TileStage {
 // One Jacobi sweep on the row-scaled system: d_new = rs - N' d_old, with
 // N' = D^-1 N stored as sval and rs = D^-1 r.  mode 0 stores d_new, mode 1
 // sets x = d_new (first refinement step, x was 0), mode 2 adds x += d_new.
-template <int K, int TW>
+template <int K, int TW, bool SORTED, bool COL16>
 __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const float* __restrict__ rs,
                                                             const float* __restrict__ d_old,
-                                                            float* __restrict__ d_new, double* x, int mode) {
-    using Stage = TileStage<K, TW, false, float, float>;
+                                                            float* __restrict__ d_new, double* x, int mode,
+                                                            int epl) {
+    using Stage = TileStage<K, TW, false, float, float, SORTED, COL16>;
     __shared__ float prod[Stage::CAP];
     __shared__ int s_ia[Stage::W + 1];
     __shared__ float s_r[TW];
@@ -203,7 +231,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const fl
     const int tid = threadIdx.x;
     const int k = A.k;
     // phase 2: a group of G2*K lanes per row
-    const int G2 = group_lanes<K>(T.E, T.rows);
+    const int G2 = group_lanes<K>(T.E, T.rows, epl);
     const int GL = G2 * K;
     const int rpp = kTileThreads / GL;
     const int npass = (T.rows + rpp - 1) / rpp;
@@ -245,7 +273,7 @@ __device__ __forceinline__ unsigned long long* norm_slot(const NormArgs& N, int 
 template <int K>
 __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const double* __restrict__ b,
                                                             const double* __restrict__ x,
-                                                            float* __restrict__ rs_out, NormArgs N) {
+                                                            float* __restrict__ rs_out, NormArgs N, int epl) {
     constexpr int TW = 1024;
     using Stage = TileStage<K, TW, true, double, double>;
     __shared__ double prod[Stage::CAP];
@@ -259,7 +287,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
     const int k = A.k;
     if (N.norms && tid < 2) s_nrm[tid] = 0ull;
     __syncthreads();
-    const int G2 = group_lanes<K>(T.E, T.rows);
+    const int G2 = group_lanes<K>(T.E, T.rows, epl);
     const int GL = G2 * K;
     const int rpp = kTileThreads / GL;
     const int npass = (T.rows + rpp - 1) / rpp;
@@ -292,7 +320,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
     if (N.norms) {
         __syncthreads();
         if (tid < 2 && s_nrm[tid] != 0ull) {
-            const int m = A.tile_member ? A.tile_member[blockIdx.x] : 0;
+            const int m = A.tile_member ? A.tile_member[xcd_tile(blockIdx.x, gridDim.x)] : 0;
             atomicMax(norm_slot(N, m, tid), s_nrm[tid]);
         }
     }
@@ -317,6 +345,72 @@ __global__ void k_jac_scale(int n, const int* __restrict__ ia, const double* __r
         const double dg = a[re - 1];
         for (int p = rs; p < re - 1; ++p) sval[p] = (float)(a[p] / dg);
         sval[re - 1] = 0.0f;
+    }
+}
+
+// Column-sorted copy of each tile's entries (setup): a bitonic sort of
+// (column, position) pairs in LDS, one workgroup per tile.  Tiles of more
+// than kSortCap entries (summed from memory by the sweep) keep storage order.
+constexpr int kSortCap = 4096;
+
+__global__ __launch_bounds__(256) void k_jac_sort_tiles(const int4* __restrict__ tiles, const int* __restrict__ ja,
+                                                        const float* __restrict__ sval, int* sja, float* ssv,
+                                                        unsigned short* sslot) {
+    __shared__ unsigned long long key[kSortCap];
+    const int4 tl = tiles[blockIdx.x];
+    const int p0 = tl.z, E = tl.w - tl.z;
+    const int tid = threadIdx.x;
+    if (E <= 0) return;
+    if (E > kSortCap) {
+        for (int i = tid; i < E; i += blockDim.x) {
+            sja[p0 + i] = ja[p0 + i];
+            ssv[p0 + i] = sval[p0 + i];
+            sslot[p0 + i] = 0;  // unused: such tiles are summed from memory
+        }
+        return;
+    }
+    int N = 1;
+    while (N < E) N <<= 1;
+    for (int i = tid; i < N; i += blockDim.x)
+        key[i] = i < E ? ((unsigned long long)(unsigned)ja[p0 + i] << 16) | (unsigned)i : ~0ull;
+    for (int size = 2; size <= N; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int i = tid; i < (N >> 1); i += blockDim.x) {
+                const int lo = 2 * i - (i & (stride - 1));
+                const int hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                const unsigned long long a = key[lo], b = key[hi];
+                if ((a > b) == asc) {
+                    key[lo] = b;
+                    key[hi] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < E; i += blockDim.x) {
+        const unsigned long long kv = key[i];
+        const int pos = (int)(kv & 0xffffull);
+        sja[p0 + i] = (int)(kv >> 16);
+        ssv[p0 + i] = sval[p0 + pos];
+        sslot[p0 + i] = (unsigned short)pos;
+    }
+}
+
+// 16-bit column copies (setup): c16 = r0 - ja in storage order and, when the
+// sorted copy exists, sc16 = r0 - sja; *bad is set when a tile holds a
+// column outside r0 -/+ 32767 (then the 16-bit form is not used).
+__global__ __launch_bounds__(256) void k_jac_col16(const int4* __restrict__ tiles, int cap, const int* __restrict__ ja,
+                                                   const int* __restrict__ sja, short* c16, short* sc16, int* bad) {
+    const int4 tl = tiles[blockIdx.x];
+    const int r0 = tl.x, p0 = tl.z, E = tl.w - tl.z;
+    if (E > cap) return;  // summed from memory with 32-bit columns
+    for (int i = threadIdx.x; i < E; i += blockDim.x) {
+        const int d = r0 - ja[p0 + i];
+        if (d < -32767 || d > 32767) atomicOr(bad, 1);
+        c16[p0 + i] = (short)d;
+        if (sja) sc16[p0 + i] = (short)(r0 - sja[p0 + i]);
     }
 }
 
@@ -422,6 +516,14 @@ static int pow2_at_least(int k) {
     return K;
 }
 
+static int free_table(TileTable& t) {
+    for (void* p : {(void*)t.tiles, (void*)t.member, (void*)t.sja, (void*)t.ssv, (void*)t.sslot, (void*)t.c16,
+                    (void*)t.sc16})
+        if (p) SPT_HIP(hipFree(p));
+    t = TileTable{};
+    return SPTRSV_OK;
+}
+
 // Workspaces and the tile table for k right-hand sides (allocates; call
 // before any capture).
 static int ensure_jacobi_ws(sp_mat_t* m, int k) {
@@ -457,8 +559,7 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
         TileTable& t = pv->jac_tt[w];
         if (t.K == K) continue;
         if (pv->jac_tiles_fixed) return fail(SPTRSV_E_CAPACITY, "batched JACOBI solves take one right-hand side");
-        if (t.tiles) SPT_HIP(hipFree(t.tiles));
-        t = TileTable{};
+        SPT_TRY(free_table(t));
         const int window = (1024 << w) / K;
         const int ntiles = (int)std::max<long long>(1, ((long long)m->nnz + window - 1) / window);
         SPT_HIP(hipMalloc(&t.tiles, 4 * sizeof(int) * (size_t)ntiles));
@@ -509,9 +610,7 @@ int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vec
     pv->margin_known = true;
     for (int w = 0; w < 2; ++w) {
         TileTable& t = pv->jac_tt[w];
-        if (t.tiles) SPT_HIP(hipFree(t.tiles));
-        if (t.member) SPT_HIP(hipFree(t.member));
-        t = TileTable{};
+        SPT_TRY(free_table(t));
         t.K = 1;
         const size_t nt = tile_member[w].size();
         if (nt == 0) continue;
@@ -525,6 +624,65 @@ int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vec
 }
 
 hipEvent_t* kernel_timer_pair(hipStream_t s);
+template <int K, int TW>
+static void launch_sweep(const TileArgs& AS, int ntiles, const float* rs, const float* cur, float* nxt, double* x,
+                         int mode, int epl, hipStream_t s) {
+    const dim3 g(ntiles), b(kTileThreads);
+    if (AS.sja) {
+        if (AS.sc16)
+            hipLaunchKernelGGL((k_jac_sweep<K, TW, true, true>), g, b, 0, s, AS, rs, cur, nxt, x, mode, epl);
+        else
+            hipLaunchKernelGGL((k_jac_sweep<K, TW, true, false>), g, b, 0, s, AS, rs, cur, nxt, x, mode, epl);
+    } else {
+        if (AS.c16)
+            hipLaunchKernelGGL((k_jac_sweep<K, TW, false, true>), g, b, 0, s, AS, rs, cur, nxt, x, mode, epl);
+        else
+            hipLaunchKernelGGL((k_jac_sweep<K, TW, false, false>), g, b, 0, s, AS, rs, cur, nxt, x, mode, epl);
+    }
+}
+
+// 16-bit column copies of tile table t when every LDS-staged tile allows
+// them (allocates; setup only).  t.c16_state: 0 unknown, 1 built, -1 refused.
+template <int TW>
+static int ensure_col16(sp_mat_t* m, TileTable& t, int K) {
+    if (t.c16_state != 0 || t.count == 0) return SPTRSV_OK;
+    const size_t nz = (size_t)std::max(1, m->nnz);
+    int* bad = nullptr;
+    SPT_HIP(hipMalloc(&t.c16, sizeof(short) * nz));
+    if (t.sja) SPT_HIP(hipMalloc(&t.sc16, sizeof(short) * nz));
+    SPT_HIP(hipMalloc(&bad, sizeof(int)));
+    SPT_HIP(hipMemset(bad, 0, sizeof(int)));
+    hipLaunchKernelGGL(k_jac_col16, dim3(t.count), dim3(256), 0, nullptr, reinterpret_cast<const int4*>(t.tiles),
+                       tile_cap<TW>() / K, m->ja, t.sja, t.c16, t.sc16, bad);
+    int h = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(&h, bad, sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(bad);
+    if (e != hipSuccess) return hip_fail(e, "16-bit columns", __FILE__, __LINE__);
+    t.c16_state = h ? -1 : 1;
+    if (h) {
+        (void)hipFree(t.c16);
+        if (t.sc16) (void)hipFree(t.sc16);
+        t.c16 = nullptr;
+        t.sc16 = nullptr;
+    }
+    return SPTRSV_OK;
+}
+
+// Column-sorted entry copy of tile table t (allocates; setup only).
+static int ensure_sorted(sp_mat_t* m, TileTable& t) {
+    MatPriv* pv = priv_of(m);
+    if (t.sja || t.count == 0) return SPTRSV_OK;
+    const size_t nz = (size_t)std::max(1, m->nnz);
+    SPT_HIP(hipMalloc(&t.sja, sizeof(int) * nz));
+    SPT_HIP(hipMalloc(&t.ssv, sizeof(float) * nz));
+    SPT_HIP(hipMalloc(&t.sslot, sizeof(unsigned short) * nz));
+    hipLaunchKernelGGL(k_jac_sort_tiles, dim3(t.count), dim3(256), 0, nullptr, reinterpret_cast<const int4*>(t.tiles),
+                       m->ja, pv->jac_sval, t.sja, t.ssv, t.sslot);
+    SPT_HIP(hipGetLastError());
+    SPT_HIP(hipDeviceSynchronize());
+    return SPTRSV_OK;
+}
 
 template <int K>
 static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, double* x,
@@ -533,10 +691,21 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     const TileTable& t0 = pv->jac_tt[0];
     const int wide = (c.lanes == 2 && pv->jac_tt[1].count > 0) ? 1 : 0;  // sweep tiles: 1024 or 2048
     const TileTable& ts = pv->jac_tt[wide];
-    TileArgs A{m->nr, k, m->ia, m->ja, pv->jac_sval, m->a, reinterpret_cast<const int4*>(t0.tiles), t0.member};
+    TileArgs A{m->nr, k, m->ia, m->ja, pv->jac_sval, m->a, reinterpret_cast<const int4*>(t0.tiles), t0.member,
+               nullptr, nullptr, nullptr, nullptr, nullptr};
     TileArgs AS = A;
     AS.tiles = reinterpret_cast<const int4*>(ts.tiles);
     AS.tile_member = ts.member;
+    if ((c.variant & 1) && ts.sja) {  // column-sorted tiles
+        AS.sja = ts.sja;
+        AS.ssv = ts.ssv;
+        AS.sslot = ts.sslot;
+    }
+    if ((c.variant & 2) && ts.c16_state == 1) {  // 16-bit columns
+        AS.c16 = ts.c16;
+        AS.sc16 = AS.sja ? ts.sc16 : nullptr;
+        if (AS.sja && !ts.sc16) AS.c16 = nullptr;  // sorted copy made after the 16-bit one
+    }
     const NormArgs none{nullptr, 1};
     const NormArgs with{pv->jac_norms, std::max(1, pv->jac_members)};
     if (t0.count == 0) return SPTRSV_OK;
@@ -544,13 +713,14 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     const unsigned first_grid = ceil_div((long long)m->nr * K, 256);
     const int sweeps = std::max(2, c.sweeps);
     const int outer = std::max(1, c.outer);
+    const int epl = c.window > 0 ? c.window : 8;  // entries a phase-2 lane sums
     float* rs = pv->jac_r;  // D^-1 r of the current step; also the first iterate
     for (int o = 0; o < outer; ++o) {
         if (o == 0)
             hipLaunchKernelGGL((k_jac_first<K>), dim3(first_grid), dim3(256), 0, s, A, b, rs);
         else
             hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b,
-                               (const double*)x, rs, none);
+                               (const double*)x, rs, none, epl);
         const float* cur = rs;
         float* nxt = pv->jac_a;
         for (int sw = 1; sw < sweeps; ++sw) {
@@ -559,11 +729,9 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
             hipEvent_t* ev = kernel_timer_pair(s);
             if (ev) SPT_HIP(hipEventRecord(ev[0], s));
             if (wide)
-                hipLaunchKernelGGL((k_jac_sweep<K, 2048>), dim3(ts.count), dim3(kTileThreads), 0, s, AS,
-                                   (const float*)rs, cur, nxt, x, mode);
+                launch_sweep<K, 2048>(AS, ts.count, rs, cur, nxt, x, mode, epl, s);
             else
-                hipLaunchKernelGGL((k_jac_sweep<K, 1024>), dim3(ts.count), dim3(kTileThreads), 0, s, AS,
-                                   (const float*)rs, cur, nxt, x, mode);
+                launch_sweep<K, 1024>(AS, ts.count, rs, cur, nxt, x, mode, epl, s);
             if (ev) SPT_HIP(hipEventRecord(ev[1], s));
             cur = nxt;
             nxt = (nxt == pv->jac_a) ? pv->jac_b : pv->jac_a;
@@ -573,7 +741,7 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     const size_t nw = (size_t)with.members * kNormReps * kNormStride;
     hipLaunchKernelGGL(k_jac_zero_norms, dim3(ceil_div((long long)nw, 256)), dim3(256), 0, s, pv->jac_norms, nw);
     hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b, (const double*)x,
-                       (float*)nullptr, with);
+                       (float*)nullptr, with, epl);
     hipLaunchKernelGGL(k_jac_certify, dim3(ceil_div(with.members, 256)), dim3(256), 0, s, pv->jac_norms,
                        with.members, pv->margin, pv->tol, pv->counters + 3);
     SPT_HIP(hipGetLastError());
@@ -585,6 +753,20 @@ int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, 
     SPT_TRY(jacobi_margin(m, &delta));
     if (!(delta > 0)) return fail(SPTRSV_E_INVALID_ARG, "JACOBI needs a strictly diagonally dominant matrix");
     SPT_TRY(ensure_jacobi_ws(m, k));
+    if (c.variant & 3) {
+        MatPriv* pv = priv_of(m);
+        const int wide = (c.lanes == 2 && pv->jac_tt[1].count > 0) ? 1 : 0;
+        TileTable& t = pv->jac_tt[wide];
+        if (c.variant & 1) SPT_TRY(ensure_sorted(m, t));
+        if ((c.variant & 2) && t.sja && t.c16_state == 1 && !t.sc16) {
+            // sorted copy made after the 16-bit one: redo the 16-bit pass for both
+            (void)hipFree(t.c16);
+            t.c16 = nullptr;
+            t.c16_state = 0;
+        }
+        const int K = pow2_at_least(k);
+        if (c.variant & 2) SPT_TRY(wide ? ensure_col16<2048>(m, t, K) : ensure_col16<1024>(m, t, K));
+    }
     switch (pow2_at_least(k)) {
         case 1: return jacobi_schedule<1>(m, c, k, b, x, s);
         case 2: return jacobi_schedule<2>(m, c, k, b, x, s);
@@ -618,7 +800,7 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
     sptrsv_config_t bc{};
     int st = SPTRSV_OK;
     int prev_outer = 16;
-    for (int sweeps : {4, 6, 8, 12, 16, 24}) {
+    for (int sweeps : {4, 6, 8, 10, 12, 14, 16, 20, 24}) {
         sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, 1, 256, 1, 0, 0, sweeps, 1, 0};
         // the smallest number of refinement steps that certifies tol/8
         bool found = false;
@@ -655,6 +837,28 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
             }
         }
         if (st != SPTRSV_OK) break;
+    }
+    // column-sorted tiles, then phase-2 entries per lane (default 8), each
+    // on the best schedule so far
+    for (int v : {-1, -2, -3, 2, 4, 16}) {
+        if (st != SPTRSV_OK || bt >= 1e30f) break;
+        sptrsv_config_t c = bc;
+        if (v < 0)
+            c.variant = -v;  // 1 sorted, 2 16-bit columns, 3 both
+        else
+            c.window = v;
+        st = jacobi_solve(m, c, k, b, x, nullptr);
+        SPT_HIP(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
+        SPT_HIP(hipEventRecord(e1, nullptr));
+        SPT_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= std::max(reps, 1);
+        if (st == SPTRSV_OK && ms < bt) {
+            bt = ms;
+            bc = c;
+        }
     }
     int zero = 0;
     (void)hipMemcpy(pv->counters + 3, &zero, sizeof(int), hipMemcpyHostToDevice);

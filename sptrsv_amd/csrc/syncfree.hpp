@@ -560,7 +560,8 @@ __global__ __launch_bounds__(WG) void k_multirow(SolveArgs A) {
 // in storage order, and reduces and publishes once at the end.  The polling
 // turn is a handful of instructions (predicated loads, compares, one ballot),
 // which keeps the detection delay close to the memory round trip.  Chunks are
-// U entries per lane; folding order per lane is the same as sf_rows'.
+// U entries per lane, the next chunk's entries loading while the current one
+// is polled; folding order per lane is the same as sf_rows'.
 template <class P, int U>
 __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, int row) {
     using word = typename P::word;
@@ -582,21 +583,35 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
     const long long t0 = wall_clock64();
     bool expired = false;
     int spins = 0;
+    // Entries of the next chunk are loaded before the current chunk is
+    // polled, so a long row pays the entry-load latency once, not per chunk.
+    int col[U], ncol[U];
+    Val val[U], nval[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = rs + e + u * L;
+        col[u] = 0;
+        if (live && q < dpos) {
+            col[u] = A.ja[q];
+            val[u] = P::load(A, q);
+        }
+    }
     for (int p = rs + e; __ballot(live && p < dpos); p += U * L) {
-        int col[U];
-        Val val[U];
         word got[U];
         bool pend[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = p + u * L;
-            pend[u] = live && q < dpos;
-            col[u] = 0;
-            got[u] = P::kUnsolved;
-            if (pend[u]) {
-                col[u] = A.ja[q];
-                val[u] = P::load(A, q);
+            const int q = p + U * L + u * L;
+            ncol[u] = 0;
+            if (live && q < dpos) {
+                ncol[u] = A.ja[q];
+                nval[u] = P::load(A, q);
             }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            pend[u] = live && p + u * L < dpos;
+            got[u] = P::kUnsolved;
         }
         while (true) {
             bool any = false;
@@ -627,6 +642,11 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (live && p + u * L < dpos) P::fold(acc, val[u], got[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            col[u] = ncol[u];
+            val[u] = nval[u];
+        }
     }
     const Acc t = group_reduce<P, 1>(acc, L);
     if (live && e == 0) {

@@ -29,7 +29,8 @@ __all__ = [
     "default_config", "set_watchdog_ms", "check_watchdog", "reserve", "generate",
     "generate_batch_member", "read_mtx_lower", "Batch", "RocsparseBaseline", "power_start",
     "power_end", "bench_solve", "device_name", "synchronize", "flush_cache", "Timer",
-    "kernel_timer", "kernel_timer_read", "dep_latency_ns", "set_tolerance", "dominance_margin",
+    "kernel_timer", "kernel_timer_read", "dep_latency_ns", "set_tolerance", "dominance_margin", "solve_graph",
+    "copy_device",
 ]
 
 
@@ -211,6 +212,14 @@ def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1,
     cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window)
     check(L.lib().sptrsv_solve(byref(mat), byref(info) if info is not None else None, precision, byref(cfg),
                                b.ptr, x.ptr, mat.nr, k, refine_steps, stream), "sptrsv_solve")
+
+
+def solve_graph(mat, info, precision: int, b: DeviceArray, x: DeviceArray, cfg: Config, k: int = 1,
+                refine_steps: int = 1, stream=None) -> None:
+    """The same solve captured once into a hipGraph (per b, x and configuration) and replayed."""
+    check(L.lib().sptrsv_solve_graph(byref(mat), byref(info) if info is not None else None, precision,
+                                     byref(cfg), b.ptr, x.ptr, mat.nr, k, refine_steps, stream),
+          "sptrsv_solve_graph")
 
 
 def reserve(mat: SpMat, k: int) -> None:

@@ -172,3 +172,49 @@ def test_batch_per_member_jacobi_choice():
     db.free()
     dx.free()
     bt.close()
+
+
+@pytest.mark.parametrize("cfg,param,k", [(2, 150000, 1), (3, 100000, 1), (4, 20, 16), (4, 20, 3)])
+def test_jacobi_tile_shapes(cfg, param, k):
+    """Every sweep shape gives the certified answer: entries in storage or
+    column-sorted order, 32- or 16-bit columns (refused where they do not
+    fit), 1024- or 2048-product tiles, phase-2 lanes summing 1..16 entries."""
+    s = I.generate(cfg, param)
+    rng = np.random.default_rng(cfg + k)
+    B = s.b if k == s.k else rng.uniform(-1, 1, s.n * k)
+    ref = O.fwd_f64(s.ia, s.ja, s.a, B, k=k)
+    d = Dev(s.ia, s.ja, s.a, analyse=False)
+    for variant in (0, 1, 2, 3):
+        for tile in (1, 2):
+            for epl in (1, 8, 16):
+                x = d.solve(B, k=k, lanes=tile, variant=variant, window=epl, sweeps=16, outer=4, **J)
+                assert relerr(x, ref) <= 1e-10, (variant, tile, epl)
+    d.close()
+
+
+def test_graph_captured_solves_replay_exactly():
+    """sptrsv_solve_graph: every replay, with eager solves of other
+    configurations in between, equals the eager solve bit for bit."""
+    s = I.generate(2, 60000)
+    d = Dev(s.ia, s.ja, s.a)
+    db = I.DeviceArray.from_numpy(s.b)
+    dx = I.DeviceArray(s.n, np.float64)
+    cfgs = [(I.PREC_F64, I._cfg(I.KERNEL_SIMPLE, 16, 256, 1, 1, 4)),
+            (I.PREC_MIXED, I._cfg(I.KERNEL_MULTIROW, 0, 256, 1, 0, 0)),
+            (I.PREC_MIXED, I._cfg(I.KERNEL_JACOBI, 2, 256, 1, 0, 0, 12, 2, 8))]
+    eager = []
+    for prec, c in cfgs:
+        I.solve(d.mat, d.info, prec, db, dx, cfg=c)
+        I.synchronize()
+        eager.append(dx.numpy())
+    for rep in range(3):
+        for j, (prec, c) in enumerate(cfgs):
+            dx.fill_bytes(0)
+            I.solve_graph(d.mat, d.info, prec, db, dx, c)
+            I.synchronize()
+            assert I.check_watchdog(d.mat) == 0
+            assert np.array_equal(dx.numpy(), eager[j]), (rep, j)
+            I.solve(d.mat, d.info, I.PREC_F64, db, dx, kernel=I.KERNEL_ORDER)   # eager work between replays
+    db.free()
+    dx.free()
+    d.close()
