@@ -97,6 +97,19 @@ def cfg_dict(c) -> dict:
     return d
 
 
+def profiled_traffic(kernel_key: str):
+    """HBM-side bytes per launch of `kernel_key` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summaries committed under profiles/ (calibrated there against a
+    copy kernel of known size), or None when no summary covers that kernel."""
+    import re
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+(\d+)", line.strip("` "))
+            if m and m.group(1) == kernel_key:
+                return int(m.group(2))
+    return None
+
+
 def cpu_baseline(s, budget_s: float):
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as O
@@ -374,10 +387,16 @@ def main() -> int:
     gnnz_m = k * nnz / (msm * 1e-3) / 1e9
     gnnz_64 = k * nnz / (ms64 * 1e-3) / 1e9
     cm = tuned[I.PREC_MIXED]
+    traffic = None
     if cm.kernel == I.KERNEL_JACOBI:
         b_launch = bytes_jacobi_sweeps(n, nnz, k, cm.sweeps, cm.outer)
         b_method = bytes_jacobi_method(n, nnz, k, cm.sweeps, cm.outer)
-        kname = "Jacobi sweep (FP32, k_jac_sweep) inside FP64 refinement"
+        kpow = 1
+        while kpow < k:
+            kpow *= 2
+        key = f"k_jac_sweep<{kpow},{1024 * max(1, cm.lanes)},{'true' if cm.variant == 1 else 'false'}>@workload{args.workload}"
+        kname = f"Jacobi sweep (FP32, {key.split('@')[0]}) inside FP64 refinement"
+        traffic = profiled_traffic(key)
     else:
         b_launch = b_method = bytes_mixed_fused(n, nnz, k)
         kname = "sync-free fused mixed-precision solve"
@@ -421,7 +440,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": kname,
             "kernel_ms": round(kmsm, 4),
             "bytes_per_launch": int(b_launch),

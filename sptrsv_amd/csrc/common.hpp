@@ -52,11 +52,6 @@ struct TileTable {
     int* sja = nullptr;
     float* ssv = nullptr;
     unsigned short* sslot = nullptr;
-    // 16-bit columns (r0 - column) in storage and sorted order; c16_state
-    // 0 not built, 1 built, -1 some tile's columns do not fit 16 bits
-    short* c16 = nullptr;
-    short* sc16 = nullptr;
-    int c16_state = 0;
 };
 
 struct MatPriv {

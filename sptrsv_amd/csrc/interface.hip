@@ -89,8 +89,6 @@ static void free_mat(sp_mat_t* m) {
             if (t.sja) (void)hipFree(t.sja);
             if (t.ssv) (void)hipFree(t.ssv);
             if (t.sslot) (void)hipFree(t.sslot);
-            if (t.c16) (void)hipFree(t.c16);
-            if (t.sc16) (void)hipFree(t.sc16);
         }
         delete pv;
     }

@@ -68,10 +68,6 @@ struct TileArgs {
     const int* sja;
     const float* ssv;
     const unsigned short* sslot;
-    // 16-bit columns, as r0 - column with r0 the tile's first row (storage
-    // and sorted order); null: 32-bit columns
-    const short* c16;
-    const short* sc16;
 };
 
 struct NormArgs {
@@ -116,7 +112,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 // right-hand side (V) and the entries; level 3 the gathered iterate (and,
 // with DIAG, the diagonals); then all LDS stores.  Products a * v[ja]
 // (diagonal included) go to prod when the tile fits.
-template <int K, int TW, bool DIAG, class V, class P, bool SORTED = false, bool COL16 = false>
+template <int K, int TW, bool DIAG, class V, class P, bool SORTED = false>
 struct TileStage {
     static constexpr int W = TW / K;                       // entries per window = max rows per tile
     static constexpr int CAP = tile_cap<TW>();
@@ -126,11 +122,11 @@ struct TileStage {
     int r0, rows, p0, E;
     bool lds;
 
-    __device__ bool run(const TileArgs& A, const V* rhs, const P* vals, const P* v, int* s_ia, double* s_dg,
-                        V* s_rhs, P* prod) {
+    __device__ bool run(const TileArgs& A, int tile, const V* rhs, const P* vals, const P* v, int* s_ia,
+                        double* s_dg, V* s_rhs, P* prod) {
         const int tid = threadIdx.x;
         const int k = A.k;
-        const int4 tl = A.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+        const int4 tl = A.tiles[tile];
         r0 = tl.x;
         rows = tl.y - tl.x;
         p0 = tl.z;
@@ -163,11 +159,11 @@ struct TileStage {
                 if (u * kTileThreads < E * K) {
                     const int e = min((tid + u * kTileThreads) / K, E - 1);
                     if (SORTED) {
-                        jv[u] = COL16 ? r0 - (int)A.sc16[p0 + e] : A.sja[p0 + e];
+                        jv[u] = A.sja[p0 + e];
                         av[u] = (P)A.ssv[p0 + e];
                         slot[u] = A.sslot[p0 + e];
                     } else {
-                        jv[u] = COL16 ? r0 - (int)A.c16[p0 + e] : A.ja[p0 + e];
+                        jv[u] = A.ja[p0 + e];
                         av[u] = vals[p0 + e];
                     }
                 }
@@ -216,17 +212,13 @@ struct TileStage {
 // One Jacobi sweep on the row-scaled system: d_new = rs - N' d_old, with
 // N' = D^-1 N stored as sval and rs = D^-1 r.  mode 0 stores d_new, mode 1
 // sets x = d_new (first refinement step, x was 0), mode 2 adds x += d_new.
-template <int K, int TW, bool SORTED, bool COL16>
-__global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const float* __restrict__ rs,
-                                                            const float* __restrict__ d_old,
-                                                            float* __restrict__ d_new, double* x, int mode,
-                                                            int epl) {
-    using Stage = TileStage<K, TW, false, float, float, SORTED, COL16>;
-    __shared__ float prod[Stage::CAP];
-    __shared__ int s_ia[Stage::W + 1];
-    __shared__ float s_r[TW];
+template <int K, int TW, bool SORTED>
+__device__ __forceinline__ void sweep_tile(const TileArgs& A, int tile, const float* __restrict__ rs,
+                                           const float* __restrict__ d_old, float* __restrict__ d_new, double* x,
+                                           int mode, int epl, float* prod, int* s_ia, float* s_r) {
+    using Stage = TileStage<K, TW, false, float, float, SORTED>;
     Stage T;
-    if (!T.run(A, rs, A.sval, d_old, s_ia, nullptr, s_r, prod)) return;
+    if (!T.run(A, tile, rs, A.sval, d_old, s_ia, nullptr, s_r, prod)) return;
     __syncthreads();
     const int tid = threadIdx.x;
     const int k = A.k;
@@ -264,6 +256,22 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const fl
     }
 }
 
+// One Jacobi sweep on the row-scaled system: d_new = rs - N' d_old, with
+// N' = D^-1 N stored as sval and rs = D^-1 r.  mode 0 stores d_new, mode 1
+// sets x = d_new (first refinement step, x was 0), mode 2 adds x += d_new.
+// One workgroup per tile.
+template <int K, int TW, bool SORTED>
+__global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const float* __restrict__ rs,
+                                                            const float* __restrict__ d_old,
+                                                            float* __restrict__ d_new, double* x, int mode,
+                                                            int epl) {
+    using Stage = TileStage<K, TW, false, float, float, SORTED>;
+    __shared__ float prod[Stage::CAP];
+    __shared__ int s_ia[Stage::W + 1];
+    __shared__ float s_r[TW];
+    sweep_tile<K, TW, SORTED>(A, xcd_tile(blockIdx.x, gridDim.x), rs, d_old, d_new, x, mode, epl, prod, s_ia, s_r);
+}
+
 __device__ __forceinline__ unsigned long long* norm_slot(const NormArgs& N, int member, int word) {
     return N.norms + ((size_t)member * kNormReps + blockIdx.x % kNormReps) * kNormStride + word;
 }
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
     __shared__ double s_b[TW];
     __shared__ unsigned long long s_nrm[2];
     Stage T;
-    if (!T.run(A, b, A.a64, x, s_ia, s_dg, s_b, prod)) return;
+    if (!T.run(A, xcd_tile(blockIdx.x, gridDim.x), b, A.a64, x, s_ia, s_dg, s_b, prod)) return;
     const int tid = threadIdx.x;
     const int k = A.k;
     if (N.norms && tid < 2) s_nrm[tid] = 0ull;
@@ -398,22 +406,6 @@ __global__ __launch_bounds__(256) void k_jac_sort_tiles(const int4* __restrict__
     }
 }
 
-// 16-bit column copies (setup): c16 = r0 - ja in storage order and, when the
-// sorted copy exists, sc16 = r0 - sja; *bad is set when a tile holds a
-// column outside r0 -/+ 32767 (then the 16-bit form is not used).
-__global__ __launch_bounds__(256) void k_jac_col16(const int4* __restrict__ tiles, int cap, const int* __restrict__ ja,
-                                                   const int* __restrict__ sja, short* c16, short* sc16, int* bad) {
-    const int4 tl = tiles[blockIdx.x];
-    const int r0 = tl.x, p0 = tl.z, E = tl.w - tl.z;
-    if (E > cap) return;  // summed from memory with 32-bit columns
-    for (int i = threadIdx.x; i < E; i += blockDim.x) {
-        const int d = r0 - ja[p0 + i];
-        if (d < -32767 || d > 32767) atomicOr(bad, 1);
-        c16[p0 + i] = (short)d;
-        if (sja) sc16[p0 + i] = (short)(r0 - sja[p0 + i]);
-    }
-}
-
 // Row tiles by nnz windows: tile t = {first row whose first entry is >=
 // t * window, the same for t + 1 (n for the last), their first entries}.
 __device__ __forceinline__ int first_row_at(const int* ia, int n, long long v) {
@@ -517,8 +509,7 @@ static int pow2_at_least(int k) {
 }
 
 static int free_table(TileTable& t) {
-    for (void* p : {(void*)t.tiles, (void*)t.member, (void*)t.sja, (void*)t.ssv, (void*)t.sslot, (void*)t.c16,
-                    (void*)t.sc16})
+    for (void* p : {(void*)t.tiles, (void*)t.member, (void*)t.sja, (void*)t.ssv, (void*)t.sslot})
         if (p) SPT_HIP(hipFree(p));
     t = TileTable{};
     return SPTRSV_OK;
@@ -627,46 +618,12 @@ hipEvent_t* kernel_timer_pair(hipStream_t s);
 template <int K, int TW>
 static void launch_sweep(const TileArgs& AS, int ntiles, const float* rs, const float* cur, float* nxt, double* x,
                          int mode, int epl, hipStream_t s) {
-    const dim3 g(ntiles), b(kTileThreads);
-    if (AS.sja) {
-        if (AS.sc16)
-            hipLaunchKernelGGL((k_jac_sweep<K, TW, true, true>), g, b, 0, s, AS, rs, cur, nxt, x, mode, epl);
-        else
-            hipLaunchKernelGGL((k_jac_sweep<K, TW, true, false>), g, b, 0, s, AS, rs, cur, nxt, x, mode, epl);
-    } else {
-        if (AS.c16)
-            hipLaunchKernelGGL((k_jac_sweep<K, TW, false, true>), g, b, 0, s, AS, rs, cur, nxt, x, mode, epl);
-        else
-            hipLaunchKernelGGL((k_jac_sweep<K, TW, false, false>), g, b, 0, s, AS, rs, cur, nxt, x, mode, epl);
-    }
-}
-
-// 16-bit column copies of tile table t when every LDS-staged tile allows
-// them (allocates; setup only).  t.c16_state: 0 unknown, 1 built, -1 refused.
-template <int TW>
-static int ensure_col16(sp_mat_t* m, TileTable& t, int K) {
-    if (t.c16_state != 0 || t.count == 0) return SPTRSV_OK;
-    const size_t nz = (size_t)std::max(1, m->nnz);
-    int* bad = nullptr;
-    SPT_HIP(hipMalloc(&t.c16, sizeof(short) * nz));
-    if (t.sja) SPT_HIP(hipMalloc(&t.sc16, sizeof(short) * nz));
-    SPT_HIP(hipMalloc(&bad, sizeof(int)));
-    SPT_HIP(hipMemset(bad, 0, sizeof(int)));
-    hipLaunchKernelGGL(k_jac_col16, dim3(t.count), dim3(256), 0, nullptr, reinterpret_cast<const int4*>(t.tiles),
-                       tile_cap<TW>() / K, m->ja, t.sja, t.c16, t.sc16, bad);
-    int h = 0;
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpy(&h, bad, sizeof(int), hipMemcpyDeviceToHost);
-    (void)hipFree(bad);
-    if (e != hipSuccess) return hip_fail(e, "16-bit columns", __FILE__, __LINE__);
-    t.c16_state = h ? -1 : 1;
-    if (h) {
-        (void)hipFree(t.c16);
-        if (t.sc16) (void)hipFree(t.sc16);
-        t.c16 = nullptr;
-        t.sc16 = nullptr;
-    }
-    return SPTRSV_OK;
+    if (AS.sja)
+        hipLaunchKernelGGL((k_jac_sweep<K, TW, true>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x,
+                           mode, epl);
+    else
+        hipLaunchKernelGGL((k_jac_sweep<K, TW, false>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt,
+                           x, mode, epl);
 }
 
 // Column-sorted entry copy of tile table t (allocates; setup only).
@@ -692,20 +649,16 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     const int wide = (c.lanes == 2 && pv->jac_tt[1].count > 0) ? 1 : 0;  // sweep tiles: 1024 or 2048
     const TileTable& ts = pv->jac_tt[wide];
     TileArgs A{m->nr, k, m->ia, m->ja, pv->jac_sval, m->a, reinterpret_cast<const int4*>(t0.tiles), t0.member,
-               nullptr, nullptr, nullptr, nullptr, nullptr};
+               nullptr, nullptr, nullptr};
     TileArgs AS = A;
     AS.tiles = reinterpret_cast<const int4*>(ts.tiles);
     AS.tile_member = ts.member;
-    if ((c.variant & 1) && ts.sja) {  // column-sorted tiles
+    if (c.variant == 1 && ts.sja) {  // column-sorted tiles
         AS.sja = ts.sja;
         AS.ssv = ts.ssv;
         AS.sslot = ts.sslot;
     }
-    if ((c.variant & 2) && ts.c16_state == 1) {  // 16-bit columns
-        AS.c16 = ts.c16;
-        AS.sc16 = AS.sja ? ts.sc16 : nullptr;
-        if (AS.sja && !ts.sc16) AS.c16 = nullptr;  // sorted copy made after the 16-bit one
-    }
+
     const NormArgs none{nullptr, 1};
     const NormArgs with{pv->jac_norms, std::max(1, pv->jac_members)};
     if (t0.count == 0) return SPTRSV_OK;
@@ -753,19 +706,10 @@ int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, 
     SPT_TRY(jacobi_margin(m, &delta));
     if (!(delta > 0)) return fail(SPTRSV_E_INVALID_ARG, "JACOBI needs a strictly diagonally dominant matrix");
     SPT_TRY(ensure_jacobi_ws(m, k));
-    if (c.variant & 3) {
+    if (c.variant == 1) {
         MatPriv* pv = priv_of(m);
         const int wide = (c.lanes == 2 && pv->jac_tt[1].count > 0) ? 1 : 0;
-        TileTable& t = pv->jac_tt[wide];
-        if (c.variant & 1) SPT_TRY(ensure_sorted(m, t));
-        if ((c.variant & 2) && t.sja && t.c16_state == 1 && !t.sc16) {
-            // sorted copy made after the 16-bit one: redo the 16-bit pass for both
-            (void)hipFree(t.c16);
-            t.c16 = nullptr;
-            t.c16_state = 0;
-        }
-        const int K = pow2_at_least(k);
-        if (c.variant & 2) SPT_TRY(wide ? ensure_col16<2048>(m, t, K) : ensure_col16<1024>(m, t, K));
+        SPT_TRY(ensure_sorted(m, pv->jac_tt[wide]));
     }
     switch (pow2_at_least(k)) {
         case 1: return jacobi_schedule<1>(m, c, k, b, x, s);
@@ -840,11 +784,11 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
     }
     // column-sorted tiles, then phase-2 entries per lane (default 8), each
     // on the best schedule so far
-    for (int v : {-1, -2, -3, 2, 4, 16}) {
+    for (int v : {-1, 2, 4, 16}) {
         if (st != SPTRSV_OK || bt >= 1e30f) break;
         sptrsv_config_t c = bc;
         if (v < 0)
-            c.variant = -v;  // 1 sorted, 2 16-bit columns, 3 both
+            c.variant = 1;  // column-sorted tiles
         else
             c.window = v;
         st = jacobi_solve(m, c, k, b, x, nullptr);

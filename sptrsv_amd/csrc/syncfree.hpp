@@ -359,6 +359,25 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
             val[u] = P::load(A, q);
         }
     }
+    // With 4-entry windows (rows longer than a window) the next window's
+    // entries load while the current one is polled, so a long row pays the
+    // entry-load latency once rather than once per window.  (16-entry
+    // windows already hold most rows; doubling their registers would cost
+    // more occupancy than it saves.)
+    constexpr bool kPrefetch = (U == 4);
+    int ncol[U];
+    Val nval[U];
+    if constexpr (kPrefetch) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = p + U * L + u * L;
+            ncol[u] = 0;
+            if (!done && q < dpos) {
+                ncol[u] = A.ja[q];
+                nval[u] = P::load(A, q);
+            }
+        }
+    }
 
     const unsigned long long gmask =
         (G == 64) ? ~0ull : (((1ull << G) - 1ull) << ((lane / G) * G));
@@ -419,12 +438,25 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
                 } else if (kc == U) {
                     p += U * L;
                     kc = 0;
+                    if constexpr (kPrefetch) {
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int q = p + u * L;
-                        if (q < dpos) {
-                            col[u] = A.ja[q];
-                            val[u] = P::load(A, q);
+                        for (int u = 0; u < U; ++u) {
+                            col[u] = ncol[u];
+                            val[u] = nval[u];
+                            const int q = p + U * L + u * L;
+                            if (q < dpos) {
+                                ncol[u] = A.ja[q];
+                                nval[u] = P::load(A, q);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int q = p + u * L;
+                            if (q < dpos) {
+                                col[u] = A.ja[q];
+                                val[u] = P::load(A, q);
+                            }
                         }
                     }
                 }
@@ -560,8 +592,8 @@ __global__ __launch_bounds__(WG) void k_multirow(SolveArgs A) {
 // in storage order, and reduces and publishes once at the end.  The polling
 // turn is a handful of instructions (predicated loads, compares, one ballot),
 // which keeps the detection delay close to the memory round trip.  Chunks are
-// U entries per lane, the next chunk's entries loading while the current one
-// is polled; folding order per lane is the same as sf_rows'.
+// U entries per lane (for U <= 4 the next chunk's entries load while the
+// current one is polled); folding order per lane is the same as sf_rows'.
 template <class P, int U>
 __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, int row) {
     using word = typename P::word;
@@ -584,7 +616,9 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
     bool expired = false;
     int spins = 0;
     // Entries of the next chunk are loaded before the current chunk is
-    // polled, so a long row pays the entry-load latency once, not per chunk.
+    // polled (chunks of up to 4 entries per lane), so a long row pays the
+    // entry-load latency once, not per chunk.
+    constexpr bool kPrefetch = (U <= 4);
     int col[U], ncol[U];
     Val val[U], nval[U];
 #pragma unroll
@@ -599,13 +633,25 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
     for (int p = rs + e; __ballot(live && p < dpos); p += U * L) {
         word got[U];
         bool pend[U];
+        if constexpr (kPrefetch) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = p + U * L + u * L;
-            ncol[u] = 0;
-            if (live && q < dpos) {
-                ncol[u] = A.ja[q];
-                nval[u] = P::load(A, q);
+            for (int u = 0; u < U; ++u) {
+                const int q = p + U * L + u * L;
+                ncol[u] = 0;
+                if (live && q < dpos) {
+                    ncol[u] = A.ja[q];
+                    nval[u] = P::load(A, q);
+                }
+            }
+        } else if (p != rs + e) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = p + u * L;
+                col[u] = 0;
+                if (live && q < dpos) {
+                    col[u] = A.ja[q];
+                    val[u] = P::load(A, q);
+                }
             }
         }
 #pragma unroll
@@ -642,10 +688,12 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (live && p + u * L < dpos) P::fold(acc, val[u], got[u]);
+        if constexpr (kPrefetch) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            col[u] = ncol[u];
-            val[u] = nval[u];
+            for (int u = 0; u < U; ++u) {
+                col[u] = ncol[u];
+                val[u] = nval[u];
+            }
         }
     }
     const Acc t = group_reduce<P, 1>(acc, L);

@@ -177,14 +177,14 @@ def test_batch_per_member_jacobi_choice():
 @pytest.mark.parametrize("cfg,param,k", [(2, 150000, 1), (3, 100000, 1), (4, 20, 16), (4, 20, 3)])
 def test_jacobi_tile_shapes(cfg, param, k):
     """Every sweep shape gives the certified answer: entries in storage or
-    column-sorted order, 32- or 16-bit columns (refused where they do not
-    fit), 1024- or 2048-product tiles, phase-2 lanes summing 1..16 entries."""
+    column-sorted order, 1024- or 2048-product tiles, phase-2 lanes summing
+    1..16 entries."""
     s = I.generate(cfg, param)
     rng = np.random.default_rng(cfg + k)
     B = s.b if k == s.k else rng.uniform(-1, 1, s.n * k)
     ref = O.fwd_f64(s.ia, s.ja, s.a, B, k=k)
     d = Dev(s.ia, s.ja, s.a, analyse=False)
-    for variant in (0, 1, 2, 3):
+    for variant in (0, 1):
         for tile in (1, 2):
             for epl in (1, 8, 16):
                 x = d.solve(B, k=k, lanes=tile, variant=variant, window=epl, sweeps=16, outer=4, **J)
