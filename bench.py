@@ -259,7 +259,13 @@ def main() -> int:
     k = s.k
     log(f"[bench] generated workload {args.workload}: n={s.n} nnz={s.nnz} k={k} ({time.time() - t0:.1f}s)")
     mat = I.allocate_memory(s.ia, s.ja, s.a)
-    info = I.multirow_analysis_base_GPU(mat, I.ALL)
+    analysis_ms = []
+    for _ in range(2):                      # GPU level-set analysis (ALL mode), first and repeat call
+        t_a = time.perf_counter()
+        info = I.multirow_analysis_base_GPU(mat, I.ALL)
+        analysis_ms.append((time.perf_counter() - t_a) * 1e3)
+        if len(analysis_ms) == 1:
+            I.free_device_analysis(None, info)
     nlevs = info.nlevs
     log(f"[bench] device {I.device_name(0)}; analysis nlevs={nlevs}")
     db = I.DeviceArray.from_numpy(s.b)
@@ -375,6 +381,29 @@ def main() -> int:
     except Exception as e:  # reported baseline only
         rocsparse = {"error": str(e)[:200]}
 
+    # rocSPARSE's own iterative triangular solve (FP64 Jacobi, host-checked
+    # residual), stopped at |r|_inf <= 1e-11, beside the certified FP32 Jacobi
+    itsv = None
+    if not args.no_extras and k == 1:
+        try:
+            ri = I.RocsparseBaseline(mat, itsv=True)
+            iters = ri.solve_itsv(db, dx, 1e-11)
+            I.synchronize()
+            timer = I.Timer()
+            timer.start()
+            reps = 3
+            for _ in range(reps):
+                iters = ri.solve_itsv(db, dx, 1e-11)
+            it_ms = timer.stop() / reps
+            x = dx.numpy()
+            itsv = {"ms": round(it_ms, 4), "iterations": iters, "analysis_ms": round(ri.analysis_ms, 3),
+                    "rel_err": float(np.max(np.abs(x - x_ref)) / np.max(np.abs(x_ref))),
+                    "note": "rocsparse_dcsritsv_solve_ex, tol 1e-11 on |b - L x|_inf, 20 iterations between tests"}
+            ri.close()
+            log(f"[bench] rocSPARSE csritsv: {it_ms:.3f} ms, {iters} iterations")
+        except Exception as e:  # reported baseline only
+            itsv = {"error": str(e)[:200]}
+
     power = None
     if not args.no_extras and I.power_start("bench", 0, 1000):
         for _ in range(max(args.steps, 50)):
@@ -432,6 +461,8 @@ def main() -> int:
             "latency_bound_fraction_f64": round(nlevs * tdep * 1e-6 / ms64, 3),
             "power_w": power,
             "rocsparse_csrsv": rocsparse,
+            "rocsparse_csritsv": itsv,
+            "analysis_ms": {"first": round(analysis_ms[0], 3), "repeat": round(analysis_ms[1], 3)},
             "plain_f64_per_rhs": per_rhs,
         },
         "roofline": {

@@ -109,6 +109,13 @@ int allocate_memory(const int* csrRowPtrL, const int* csrColIdxL,
 int allocate_memory_f32(const int* csrRowPtrL, const int* csrColIdxL,
                         const float* csrValL, int n, sp_mat_t* gpu_L, MODE mode);
 
+/* Upper-triangular U (diagonal first in every row, other columns in (i, n)):
+ * no counterpart in the reference (which solves L only).  U is stored as the
+ * lower triangle of the reversed index order, so every entry point below --
+ * analysis, the three solvers, mixed precision, Jacobi, autotuning, captured
+ * solves -- then solves U x = b with b and x in the caller's order. */
+int allocate_memory_upper(const int* csrRowPtrU, const int* csrColIdxU, const double* csrValU, int n,
+                          sp_mat_t* gpu_U, MODE mode);
 /* interface.h:7 / interface.cu:28-38.  Runs the analysis for info->mode
  * (set by the caller) into the caller-allocated *info.  Returns
  * SPTRSV_E_NO_ANALYSIS_NEEDED (-1) for SIMPLE, like the reference. */
@@ -290,6 +297,13 @@ void sptrsv_batch_destroy(sptrsv_batch_t* bt);
 typedef struct sptrsv_rocsparse sptrsv_rocsparse_t;
 int csr_L_rocsparse_create(sp_mat_t* mat, sptrsv_rocsparse_t** out, float* analysis_ms);
 int csr_L_solve_rocsparse(sptrsv_rocsparse_t* h, const double* b, double* x, hipStream_t stream);
+/* rocSPARSE csritsv (FP64 Jacobi with a residual stopping test, host
+ * driven) as a second baseline: x starts at 0; iterates until
+ * |b - L x|_inf <= tol or max_iter iterations (free_iter iterations between
+ * two tests); *iters gets the iterations done. */
+int csr_L_rocsparse_itsv_create(sp_mat_t* mat, sptrsv_rocsparse_t** out, float* analysis_ms);
+int csr_L_solve_rocsparse_itsv(sptrsv_rocsparse_t* h, const double* b, double* x, double tol,
+                               int max_iter, int free_iter, int* iters, hipStream_t stream);
 void csr_L_rocsparse_destroy(sptrsv_rocsparse_t* h);
 
 /* ---- power (replaces nvmlAPIRun/nvmlAPIEnd, nvmlPower.hpp:17-18) ------ */

@@ -77,6 +77,7 @@ _PROTOS = {
     "sptrsv_last_error": (c_char_p, []),
     "allocate_memory": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
     "allocate_memory_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
+    "allocate_memory_upper": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
     "run_analysis": (c_int, [POINTER(AnalysisInfo), POINTER(SpMat)]),
     "run_solver": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int]),
     "free_device_analysis": (None, [POINTER(SpMat), POINTER(AnalysisInfo)]),
@@ -111,6 +112,9 @@ _PROTOS = {
     "sptrsv_batch_destroy": (None, [c_void_p]),
     "csr_L_rocsparse_create": (c_int, [POINTER(SpMat), POINTER(c_void_p), POINTER(c_float)]),
     "csr_L_solve_rocsparse": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "csr_L_rocsparse_itsv_create": (c_int, [POINTER(SpMat), POINTER(c_void_p), POINTER(c_float)]),
+    "csr_L_solve_rocsparse_itsv": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int, c_int, _P_INT,
+                                           c_void_p]),
     "csr_L_rocsparse_destroy": (None, [c_void_p]),
     "amdsmiAPIRun": (c_int, [c_char_p, c_int, c_int]),
     "amdsmiAPIEnd": (c_float, []),

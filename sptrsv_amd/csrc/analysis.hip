@@ -26,6 +26,8 @@
 
 namespace sptrsv {
 
+unsigned lds_pad_for(int cu_waves, int wg);
+
 // Tile of the level pass from the integer average nnz/n, as
 // aux_call_kernel_analysis_L_coop_groups picks it (analysis_csr.cu:215-240):
 // 0-1 -> 1, 2 -> 2, 3-5 -> 4, 6-11 -> 8, 12-23 -> 16, else 32.
@@ -395,6 +397,22 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
     SPT_HIP(hipMalloc(&info->level, sizeof(int) * (size_t)n));
     int lanes = 2, wg = 256;
     solve_config_for_analysis(m, &lanes, &wg);
+    // Levels are integers, so every correct pass gives the same array; the
+    // kernel is chosen for speed.  Rows of moderate length (mean >= 6, none
+    // over 256 entries) take the lean one-row-per-wave pass with its lanes
+    // covering a typical row and 4 resident waves per CU, which keeps the
+    // dependency hand-off short; other matrices keep the reference's tile
+    // rule on the general engine.
+    int kind = KK_SIMPLE;
+    unsigned dyn_lds = 0;
+    const int avg = n > 0 ? m->nnz / n : 0;
+    if (avg >= 6 && pv->max_row <= 256) {
+        kind = KK_FAST_SIMPLE;
+        lanes = 4;
+        while (lanes < avg && lanes < kWave) lanes <<= 1;
+        wg = 256;
+        dyn_lds = lds_pad_for(4, wg);
+    }
     {
         SolveArgs A{};
         A.n = n;
@@ -409,11 +427,12 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
         A.ticket = pv->counters;
         A.err = pv->counters + 1;
         A.wd_ticks = pv->watchdog_ticks;
+        A.dyn_lds = dyn_lds;
         hipLaunchKernelGGL((k_prepare<uint32_t>), dim3(grid_for(n)), dim3(256), 0, s,
                            reinterpret_cast<uint32_t*>(info->level), (size_t)n, 0u, pv->counters);
         SPT_HIP(hipGetLastError());
         int u = (pv->max_row > 2 * lanes) ? 4 : 1;
-        hipError_t e = launch_solve<PolLevel, 1>(KK_SIMPLE, 1, wg, u, A, s);
+        hipError_t e = launch_solve<PolLevel, 1>(kind, 1, wg, u, A, s);
         if (e != hipSuccess) return hip_fail(e, "level pass", __FILE__, __LINE__);
     }
     // 2. nlevs

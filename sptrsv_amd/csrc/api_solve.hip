@@ -415,13 +415,60 @@ static int check_solve_args(sp_mat_t* m, int precision, const void* b, void* x, 
     return SPTRSV_OK;
 }
 
+// dst row i = src row n-1-i (k values of `T` per row).
+template <class T>
+__global__ void k_reverse_rows(const T* __restrict__ src, T* __restrict__ dst, long long n, int k) {
+    const long long total = n * k;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long i = t / k;
+        const int r = (int)(t - i * k);
+        dst[t] = src[(n - 1 - i) * k + r];
+    }
+}
+
+static int reverse_rows(const void* src, void* dst, int n, int k, size_t esz, hipStream_t s) {
+    const unsigned g = stream_grid((size_t)n * k);
+    if (esz == 8)
+        hipLaunchKernelGGL(k_reverse_rows<double>, dim3(g), dim3(256), 0, s, static_cast<const double*>(src),
+                           static_cast<double*>(dst), (long long)n, k);
+    else
+        hipLaunchKernelGGL(k_reverse_rows<float>, dim3(g), dim3(256), 0, s, static_cast<const float*>(src),
+                           static_cast<float*>(dst), (long long)n, k);
+    SPT_HIP(hipGetLastError());
+    return SPTRSV_OK;
+}
+
+// solve_impl in the caller's index order: an upper-triangular matrix (kept as
+// the reversed lower triangle) has b reversed on the way in and x on the way
+// out.  The two workspaces are allocated on the first such solve.
+static int solve_oriented(sp_mat_t* m, dfr_analysis_info_t* info, int precision, const sptrsv_config_t& c,
+                          const void* b, void* x, int k, int refine_steps, hipStream_t s) {
+    MatPriv* pv = priv_of(m);
+    if (!pv->reversed) return solve_impl(m, info, precision, c, b, x, k, refine_steps, s);
+    const size_t esz = precision == SPTRSV_PREC_F32 ? sizeof(float) : sizeof(double);
+    const size_t bytes = std::max<size_t>((size_t)m->nr * k * esz, 8);
+    if (pv->rev_bytes < bytes) {
+        if (pv->rev_b) SPT_HIP(hipFree(pv->rev_b));
+        if (pv->rev_x) SPT_HIP(hipFree(pv->rev_x));
+        pv->rev_b = pv->rev_x = nullptr;
+        pv->rev_bytes = 0;
+        SPT_HIP(hipMalloc(&pv->rev_b, bytes));
+        SPT_HIP(hipMalloc(&pv->rev_x, bytes));
+        pv->rev_bytes = bytes;
+    }
+    SPT_TRY(reverse_rows(b, pv->rev_b, m->nr, k, esz, s));
+    SPT_TRY(solve_impl(m, info, precision, c, pv->rev_b, pv->rev_x, k, refine_steps, s));
+    return reverse_rows(pv->rev_x, x, m->nr, k, esz, s);
+}
+
 int solve_checked(sp_mat_t* m, dfr_analysis_info_t* info, int precision, const sptrsv_config_t* cfg,
                   const void* b, void* x, int n, int k, int refine_steps, hipStream_t s) {
     SPT_TRY(check_solve_args(m, precision, b, x, n, k, refine_steps));
     if (n == 0) return SPTRSV_OK;
     sptrsv_config_t c;
     SPT_TRY(resolve_config(m, info, precision, k, cfg, &c));
-    return solve_impl(m, info, precision, c, b, x, k, refine_steps, s);
+    return solve_oriented(m, info, precision, c, b, x, k, refine_steps, s);
 }
 
 }  // namespace sptrsv
@@ -449,14 +496,14 @@ int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, 
     if (it == pv->graphs.end()) {
         // one eager solve sets up every workspace and table (no allocation
         // may happen inside the capture)
-        SPT_TRY(solve_impl(mat, info, precision, c, b, x, k, refine_steps, stream));
+        SPT_TRY(solve_oriented(mat, info, precision, c, b, x, k, refine_steps, stream));
         SPT_HIP(hipStreamSynchronize(stream));
         if (!pv->cap_stream) SPT_HIP(hipStreamCreate(&pv->cap_stream));
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         SPT_HIP(hipStreamBeginCapture(pv->cap_stream, hipStreamCaptureModeThreadLocal));
         g_capturing = true;
-        const int st = solve_impl(mat, info, precision, c, b, x, k, refine_steps, pv->cap_stream);
+        const int st = solve_oriented(mat, info, precision, c, b, x, k, refine_steps, pv->cap_stream);
         g_capturing = false;
         hipError_t e = hipStreamEndCapture(pv->cap_stream, &g);
         if (st != SPTRSV_OK) {

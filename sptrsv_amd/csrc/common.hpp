@@ -74,6 +74,12 @@ struct MatPriv {
     double tol = 1e-10;          // certified relative accuracy
     double margin = 0.0;         // min_i |L_ii| - sum_j |L_ij|
     bool margin_known = false;
+    // upper-triangular matrix stored as the lower triangle of the reversed
+    // index order: solves reverse b into rev_b and the result back from rev_x
+    bool reversed = false;
+    void* rev_b = nullptr;
+    void* rev_x = nullptr;
+    size_t rev_bytes = 0;
     float* jac_r = nullptr;      // FP32 row-scaled residual D^-1 r, n*k
     float* jac_sval = nullptr;   // FP32 a_ij / a_ii (0 on the diagonal), nnz
     float* jac_a = nullptr;      // Jacobi iterates (ping-pong), n*k each

@@ -78,6 +78,8 @@ static void free_mat(sp_mat_t* m) {
         if (pv->corr) (void)hipFree(pv->corr);
         if (pv->resid) (void)hipFree(pv->resid);
         if (pv->trace) (void)hipFree(pv->trace);
+        if (pv->rev_b) (void)hipFree(pv->rev_b);
+        if (pv->rev_x) (void)hipFree(pv->rev_x);
         for (float* p : {pv->jac_r, pv->jac_a, pv->jac_b, pv->jac_sval})
             if (p) (void)hipFree(p);
         if (pv->jac_norms) (void)hipFree(pv->jac_norms);
@@ -315,6 +317,46 @@ int allocate_memory_f32(const int* csrRowPtrL, const int* csrColIdxL, const floa
                         sp_mat_t* gpu_L, MODE mode) {
     (void)mode;
     return upload(csrRowPtrL, csrColIdxL, csrValL, n, gpu_L);
+}
+
+// Upper triangle with the diagonal first in every row and the other columns
+// in (i, n): uploaded as the lower triangle of the reversed index order,
+// row n-1-i holding columns n-1-j in reversed entry order (diagonal last).
+int allocate_memory_upper(const int* csrRowPtrU, const int* csrColIdxU, const double* csrValU, int n,
+                          sp_mat_t* gpu_U, MODE mode) {
+    (void)mode;
+    if (!gpu_U) return fail(SPTRSV_E_INVALID_ARG, "null sp_mat_t");
+    if (n < 0 || (n > 0 && (!csrRowPtrU || !csrColIdxU || !csrValU))) return fail(SPTRSV_E_INVALID_ARG, "null CSR");
+    if (n == 0) return upload(csrRowPtrU, csrColIdxU, csrValU, n, gpu_U);
+    const long long base = csrRowPtrU[0];
+    if (base < 0) return fail(SPTRSV_E_INVALID_CSR, "negative row pointer");
+    const long long nnz = (long long)csrRowPtrU[n] - base;
+    if (nnz < n || nnz > 0x7fffffffLL) return fail(SPTRSV_E_INVALID_CSR, "bad row pointers");
+    std::vector<int> ia(n + 1), ja((size_t)nnz);
+    std::vector<double> a((size_t)nnz);
+    long long w = 0;
+    ia[0] = 0;
+    for (int r = 0; r < n; ++r) {
+        const int i = n - 1 - r;
+        const long long s = (long long)csrRowPtrU[i] - base, e = (long long)csrRowPtrU[i + 1] - base;
+        if (e <= s || s < 0 || e > nnz)
+            return fail(SPTRSV_E_INVALID_CSR, "row " + std::to_string(i) + " has no diagonal entry");
+        if (csrColIdxU[base + s] != i)
+            return fail(SPTRSV_E_INVALID_CSR, "row " + std::to_string(i) + ": first entry is not the diagonal");
+        for (long long p = e - 1; p >= s; --p) {
+            const int j = csrColIdxU[base + p];
+            if (p > s && (j <= i || j >= n))
+                return fail(SPTRSV_E_INVALID_CSR, "row " + std::to_string(i) + ": column " + std::to_string(j) +
+                                                      " outside the strict upper triangle");
+            ja[(size_t)w] = n - 1 - j;
+            a[(size_t)w] = csrValU[base + p];
+            ++w;
+        }
+        ia[r + 1] = (int)w;
+    }
+    SPT_TRY(upload(ia.data(), ja.data(), a.data(), n, gpu_U));
+    priv_of(gpu_U)->reversed = true;
+    return SPTRSV_OK;
 }
 
 void free_device_analysis(sp_mat_t* gpu_L, dfr_analysis_info_t* info) {

@@ -109,6 +109,23 @@ def allocate_memory_f32(ia, ja, a, n: int | None = None, mode: int = ALL) -> SpM
     return m
 
 
+def allocate_memory_upper(ia, ja, a, n: int | None = None, mode: int = ALL) -> SpMat:
+    """Upload a host CSR upper triangle (diagonal first in every row).
+
+    The matrix is kept as the lower triangle of the reversed index order, so
+    every analysis and solve call on it solves U x = b with b and x in the
+    caller's order.  No counterpart in the reference, which solves L only.
+    """
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja, np.int32)
+    a = np.ascontiguousarray(a, np.float64)
+    n = len(ia) - 1 if n is None else n
+    m = SpMat()
+    check(L.lib().allocate_memory_upper(_ptr(ia), _ptr(ja), _ptr(a), n, byref(m), mode),
+          "allocate_memory_upper")
+    return m
+
+
 def run_analysis(info: AnalysisInfo, mat: SpMat) -> int:
     """interface.h:7: returns -1 for SIMPLE (no analysis), like the reference."""
     st = L.lib().run_analysis(byref(info), byref(mat))
@@ -374,16 +391,26 @@ class Batch:
 
 
 class RocsparseBaseline:
-    """rocSPARSE csrsv on the same device matrix (solver.h:8 counterpart)."""
+    """rocSPARSE csrsv (or, with itsv=True, csritsv: FP64 Jacobi with a residual
+    stopping test) on the same device matrix (solver.h:8 counterpart)."""
 
-    def __init__(self, mat: SpMat):
+    def __init__(self, mat: SpMat, itsv: bool = False):
         h = c_void_p()
         ms = c_float()
-        check(L.lib().csr_L_rocsparse_create(byref(mat), byref(h), byref(ms)), "csr_L_rocsparse_create")
-        self.h, self.analysis_ms = h, ms.value
+        fn = L.lib().csr_L_rocsparse_itsv_create if itsv else L.lib().csr_L_rocsparse_create
+        check(fn(byref(mat), byref(h), byref(ms)), "rocSPARSE create")
+        self.h, self.analysis_ms, self.itsv = h, ms.value, itsv
 
     def solve(self, b: DeviceArray, x: DeviceArray, stream=None) -> None:
         check(L.lib().csr_L_solve_rocsparse(self.h, b.ptr, x.ptr, stream), "csr_L_solve_rocsparse")
+
+    def solve_itsv(self, b: DeviceArray, x: DeviceArray, tol: float, max_iter: int = 1000, free_iter: int = 20,
+                   stream=None) -> int:
+        """Iterations done (< max_iter when |b - L x|_inf <= tol was reached)."""
+        it = c_int()
+        check(L.lib().csr_L_solve_rocsparse_itsv(self.h, b.ptr, x.ptr, tol, max_iter, free_iter, byref(it), stream),
+              "csr_L_solve_rocsparse_itsv")
+        return it.value
 
     def close(self) -> None:
         if self.h is not None:

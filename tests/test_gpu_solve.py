@@ -428,6 +428,15 @@ def test_rocsparse_baseline(device):
     I.synchronize()
     assert relerr(dx.numpy(), ref) <= 1e-12
     rb.close()
+    # csritsv: FP64 Jacobi to |b - L x|_inf <= 1e-12 (delta = 1 for this matrix)
+    ri = I.RocsparseBaseline(mat, itsv=True)
+    iters = ri.solve_itsv(db, dx, 1e-12, max_iter=2000)
+    I.synchronize()
+    assert 0 < iters < 2000
+    assert relerr(dx.numpy(), ref) <= 1e-10
+    ri.close()
+    db.free()
+    dx.free()
     I.free_device_analysis(mat, None)
 
 

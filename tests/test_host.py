@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_status_strings():
     lib = L.lib()
-    for st in range(0, -11, -1):
+    for st in range(0, -12, -1):
         assert lib.sptrsv_status_string(st).decode() not in ("", "unknown status")
     assert lib.sptrsv_status_string(-99).decode() == "unknown status"
 
@@ -49,6 +49,18 @@ def test_allocate_rejects_invalid_csr(ia, ja, why):
     # validation happens on the host before any device call
     with pytest.raises(L.SptrsvError) as e:
         I.allocate_memory(np.array(ia), np.array(ja), np.ones(len(ja)))
+    assert e.value.status == L.E_INVALID_CSR, why
+
+
+@pytest.mark.parametrize("ia,ja,why", [
+    ([0, 2, 3], [1, 0, 1], "diagonal not first"),
+    ([0, 1, 1], [0], "empty row"),
+    ([0, 1, 3], [0, 0, 1], "lower entry"),
+    ([0, 2, 3], [0, 2, 1], "column past the last row"),
+])
+def test_allocate_upper_rejects_invalid_csr(ia, ja, why):
+    with pytest.raises(L.SptrsvError) as e:
+        I.allocate_memory_upper(np.array(ia), np.array(ja), np.ones(len(ja)))
     assert e.value.status == L.E_INVALID_CSR, why
 
 
