@@ -116,6 +116,12 @@ int allocate_memory_f32(const int* csrRowPtrL, const int* csrColIdxL,
  * solves -- then solves U x = b with b and x in the caller's order. */
 int allocate_memory_upper(const int* csrRowPtrU, const int* csrColIdxU, const double* csrValU, int n,
                           sp_mat_t* gpu_U, MODE mode);
+/* Transpose: no counterpart in the reference.  Builds, on the device, a new
+ * handle *dst that solves A^T x = b, where A is what src solves (L, or U from
+ * allocate_memory_upper) -- e.g. the L^T half of an incomplete-Cholesky
+ * preconditioner from the L already uploaded.  dst is independent of src (its
+ * own arrays; analyse it and free it like any other handle). */
+int sptrsv_transpose(sp_mat_t* src, sp_mat_t* dst);
 /* interface.h:7 / interface.cu:28-38.  Runs the analysis for info->mode
  * (set by the caller) into the caller-allocated *info.  Returns
  * SPTRSV_E_NO_ANALYSIS_NEEDED (-1) for SIMPLE, like the reference. */

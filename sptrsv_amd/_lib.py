@@ -78,6 +78,7 @@ _PROTOS = {
     "allocate_memory": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
     "allocate_memory_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
     "allocate_memory_upper": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
+    "sptrsv_transpose": (c_int, [POINTER(SpMat), POINTER(SpMat)]),
     "run_analysis": (c_int, [POINTER(AnalysisInfo), POINTER(SpMat)]),
     "run_solver": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int]),
     "free_device_analysis": (None, [POINTER(SpMat), POINTER(AnalysisInfo)]),

@@ -40,7 +40,8 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}",
              f"-I{ROCM / 'include'}", "-D__HIP_PLATFORM_AMD__"]
 
 HIP_SOURCES = ["inst_f64.hip", "inst_f32.hip", "inst_mixed.hip", "inst_level.hip",
-               "api_solve.hip", "analysis.hip", "interface.hip", "batch.hip", "iterative.hip"]
+               "api_solve.hip", "analysis.hip", "interface.hip", "batch.hip", "iterative.hip",
+               "transpose.hip"]
 HOST_HIP_SOURCES = ["power.cpp", "rocsparse_baseline.cpp"]   # need HIP headers, no kernels
 CXX_SOURCES = ["mmio.cpp", "generate.cpp"]
 

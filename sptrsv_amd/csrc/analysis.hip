@@ -144,7 +144,7 @@ __global__ void k_scan_apply(const int* in, int* out, long long n, const int* of
 }
 
 // out[i] = sum_{j<i} in[j]; in and out may alias.
-static int exclusive_scan(const int* in, int* out, long long n, hipStream_t s) {
+int exclusive_scan(const int* in, int* out, long long n, hipStream_t s) {
     if (n <= 0) return SPTRSV_OK;
     long long blocks = (n + kScanTile - 1) / kScanTile;
     if (blocks == 1) {
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const int* __restrict
 
 // Sorts (keys, vals) by key, stably; max_key bounds the keys.  The result is
 // left in (keys, vals); tk/tv are scratch of the same length.
-static int radix_sort_pairs(int* keys, int* vals, int* tk, int* tv, int n, int max_key,
+int radix_sort_pairs(int* keys, int* vals, int* tk, int* tv, int n, int max_key,
                             hipStream_t s) {
     if (n <= 1) return SPTRSV_OK;
     int bits = 0;

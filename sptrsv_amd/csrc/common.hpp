@@ -101,6 +101,12 @@ struct MatPriv {
 
 MatPriv* priv_of(sp_mat_t* m);
 int ensure_priv(sp_mat_t* m);
+void free_mat(sp_mat_t* m);  // device arrays and private state; zeroes *m
+
+// out[i] = sum_{j<i} in[j] on the device (in and out may alias)
+int exclusive_scan(const int* in, int* out, long long n, hipStream_t s);
+// stable LSD radix sort of (keys, vals) by key in [0, max_key]; tk/tv scratch
+int radix_sort_pairs(int* keys, int* vals, int* tk, int* tv, int n, int max_key, hipStream_t s);
 
 // ---- launch helpers ----------------------------------------------------------
 inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }

@@ -64,7 +64,7 @@ int ensure_priv(sp_mat_t* m) {
     return SPTRSV_OK;
 }
 
-static void free_mat(sp_mat_t* m) {
+void free_mat(sp_mat_t* m) {
     if (!m) return;
     if (m->ia) (void)hipFree(m->ia);
     if (m->ja) (void)hipFree(m->ja);

@@ -30,7 +30,7 @@ __all__ = [
     "generate_batch_member", "read_mtx_lower", "Batch", "RocsparseBaseline", "power_start",
     "power_end", "bench_solve", "device_name", "synchronize", "flush_cache", "Timer",
     "kernel_timer", "kernel_timer_read", "dep_latency_ns", "set_tolerance", "dominance_margin", "solve_graph",
-    "copy_device",
+    "copy_device", "allocate_memory_upper", "transpose", "matrix_arrays",
 ]
 
 
@@ -124,6 +124,26 @@ def allocate_memory_upper(ia, ja, a, n: int | None = None, mode: int = ALL) -> S
     check(L.lib().allocate_memory_upper(_ptr(ia), _ptr(ja), _ptr(a), n, byref(m), mode),
           "allocate_memory_upper")
     return m
+
+
+def transpose(mat: SpMat) -> SpMat:
+    """A new handle that solves A^T x = b for the A that `mat` solves, built on the device."""
+    t = SpMat()
+    check(L.lib().sptrsv_transpose(byref(mat), byref(t)), "sptrsv_transpose")
+    return t
+
+
+def matrix_arrays(mat: SpMat) -> dict:
+    """Copies a handle's stored CSR (the lower triangle the kernels read) to numpy."""
+    n, nnz = mat.nr, mat.nnz
+    out = {"ia": _dev_int_array(mat.ia, n + 1), "ja": _dev_int_array(mat.ja, nnz)}
+    for name, dt in (("a", np.float64), ("a32", np.float32)):
+        arr = np.empty(nnz, dt)
+        ptr = getattr(mat, name)
+        if ptr and nnz:
+            check(L.lib().sptrsv_memcpy_dtoh(_ptr(arr), c_void_p(ptr), arr.nbytes), "memcpy dtoh")
+        out[name] = arr if ptr else None
+    return out
 
 
 def run_analysis(info: AnalysisInfo, mat: SpMat) -> int:

@@ -148,3 +148,81 @@ def test_upper_unsorted_columns_and_host_interface(device):
     bl = np.array([1.0, 2.0, 3.0])
     assert relerr(I.run_solver(m, None, bl), O.fwd_f64(li, lj, la, bl)) <= 1e-12
     I.free_device_analysis(m, None)
+
+
+# ---- transposed solves: a handle built on the device from an uploaded one ------------
+
+def stored_reversed_transpose(ia, ja, a, n):
+    """P A^T P in CSR with sorted columns: what a transpose handle must store for lower A."""
+    Am = sp.csr_matrix((a, ja, ia), shape=(n, n))
+    S = sp.csr_matrix(sp.csr_matrix(Am.T)[::-1][:, ::-1])
+    S.sort_indices()
+    return S
+
+
+@pytest.mark.parametrize("cfg,param", [(1, 0), (2, 30000), (3, 30000), (4, 20)])
+def test_transpose_structure_is_exact(device, cfg, param):
+    s = I.generate(cfg, param)
+    mat = I.allocate_memory(s.ia, s.ja, s.a)
+    t = I.transpose(mat)
+    got = I.matrix_arrays(t)
+    S = stored_reversed_transpose(s.ia, s.ja, s.a, s.n)
+    assert np.array_equal(got["ia"], S.indptr)
+    assert np.array_equal(got["ja"], S.indices)
+    assert np.array_equal(got["a"], S.data)
+    assert np.array_equal(got["a32"], S.data.astype(np.float32))
+    I.free_device_analysis(t, None)
+    I.free_device_analysis(mat, None)
+
+
+@pytest.mark.parametrize("cfg,param", [(2, 30000), (3, 30000)])
+def test_transpose_solves(device, cfg, param):
+    s = I.generate(cfg, param)
+    Lm = sp.csr_matrix((s.a, s.ja, s.ia), shape=(s.n, s.n))
+    LT = sp.csr_matrix(Lm.T)
+    LT.sort_indices()
+    mat = I.allocate_memory(s.ia, s.ja, s.a)
+    t = I.transpose(mat)
+    info = I.multirow_analysis_base_GPU(t, I.ALL)
+    ref = references(LT, s.b, 1)
+    for kern in KERNELS:
+        for prec, tol in ((I.PREC_F64, 1e-12), (I.PREC_MIXED, 1e-10)):
+            db = I.DeviceArray.from_numpy(s.b)
+            dx = I.DeviceArray(s.n, np.float64)
+            I.solve(t, info, prec, db, dx, kernel=kern)
+            assert I.check_watchdog(t) == 0, L.last_error()
+            assert relerr(dx.numpy(), ref) <= tol, (kern, prec)
+    # the transpose of the transpose solves L again
+    tt = I.transpose(t)
+    assert np.array_equal(I.matrix_arrays(tt)["ja"], s.ja)
+    assert relerr(I.run_solver(tt, None, s.b), O.fwd_f64(s.ia, s.ja, s.a, s.b)) <= 1e-12
+    for m, inf in ((tt, None), (t, info), (mat, None)):
+        I.free_device_analysis(m, inf)
+
+
+def test_transpose_of_upper_and_f32_handles(device):
+    s = I.generate(2, 20000)
+    U = reversed_of(s)                       # upper; U^T is lower
+    up = I.allocate_memory_upper(U.indptr, U.indices, U.data)
+    t = I.transpose(up)
+    UT = sp.csr_matrix(U.T)
+    UT.sort_indices()
+    assert np.array_equal(I.matrix_arrays(t)["ja"], UT.indices)  # stored as is: no reversal
+    b = np.random.default_rng(11).uniform(-1, 1, s.n)
+    ref = O.fwd_f64(UT.indptr, UT.indices, UT.data, b)
+    assert relerr(I.run_solver(t, None, b), ref) <= 1e-12
+    # an FP32 upload carries only FP32 values; its transpose solves in FP32
+    m32 = I.allocate_memory_f32(s.ia, s.ja, s.a.astype(np.float32))
+    t32 = I.transpose(m32)
+    got = I.matrix_arrays(t32)
+    assert got["a"] is None
+    S = stored_reversed_transpose(s.ia, s.ja, s.a.astype(np.float32).astype(np.float64), s.n)
+    assert np.array_equal(got["a32"], S.data.astype(np.float32))
+    Lm = sp.csr_matrix((s.a, s.ja, s.ia), shape=(s.n, s.n))
+    ref = references(sp.csr_matrix(Lm.T), b, 1)
+    db = I.DeviceArray.from_numpy(b.astype(np.float32))
+    dx = I.DeviceArray(s.n, np.float32)
+    I.solve(t32, None, I.PREC_F32, db, dx, kernel=I.KERNEL_SIMPLE)
+    assert relerr(dx.numpy().astype(np.float64), ref) <= 1e-4
+    for m in (t32, m32, t, up):
+        I.free_device_analysis(m, None)
