@@ -372,6 +372,24 @@ char* mm_typecode_to_str_r(const MM_typecode matcode, char* buf);
  * *ia, *ja, *a are malloc'ed; release with sptrsv_free_host. */
 int sptrsv_read_mtx_lower(const char* path, int unit, int* n, int* nnz,
                           int** ia, int** ja, double** a);
+/* The entry section of main.cu:131-155 read by nthreads threads (<= 0: the
+ * host's cores, at most 16) over line-aligned chunks: 0-based COO in file
+ * order, not mirrored; pattern files give 1.0; *symmetric is 1 for symmetric
+ * and hermitian files.  Same entries, errors and values as a sequential read.
+ * Arrays are malloc'ed (sptrsv_free_host). */
+int sptrsv_read_mtx_coo(const char* path, int nthreads, int* n, int* nz, int* symmetric,
+                        int** row, int** col, double** val);
+/* main.cu:124-246 on the device: COO (host arrays, file order) -> CSR of A
+ * with symmetric mirroring -> lower triangle with an explicit diagonal, the
+ * same matrix sptrsv_read_mtx_lower builds on the host, uploaded as *gpu_L
+ * (an allocate_memory handle). */
+int sptrsv_coo_to_lower(const int* row, const int* col, const double* val, int nz, int n,
+                        int symmetric, int unit, sp_mat_t* gpu_L);
+/* Binary CSR cache (no counterpart in the reference): magic, sizes and an
+ * FNV-1a checksum, then ia, ja, a.  A load that fails the checksum is
+ * SPTRSV_E_IO.  Loaded arrays are malloc'ed (sptrsv_free_host). */
+int sptrsv_save_csr(const char* path, int n, const int* ia, const int* ja, const double* a);
+int sptrsv_load_csr(const char* path, int* n, int* nnz, int** ia, int** ja, double** a);
 void sptrsv_free_host(void* p);
 
 /* ---- synthetic inputs of the benchmark workloads ----------------------- */

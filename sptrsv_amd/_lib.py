@@ -127,6 +127,10 @@ _PROTOS = {
     "mm_is_valid": (c_int, [c_char_p]),
     "mm_typecode_to_str_r": (c_char_p, [c_char_p, c_char_p]),
     "sptrsv_read_mtx_lower": (c_int, [c_char_p, c_int, _P_INT, _P_INT, _P_PINT, _P_PINT, _P_PDBL]),
+    "sptrsv_read_mtx_coo": (c_int, [c_char_p, c_int, _P_INT, _P_INT, _P_INT, _P_PINT, _P_PINT, _P_PDBL]),
+    "sptrsv_coo_to_lower": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, POINTER(SpMat)]),
+    "sptrsv_save_csr": (c_int, [c_char_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "sptrsv_load_csr": (c_int, [c_char_p, _P_INT, _P_INT, _P_PINT, _P_PINT, _P_PDBL]),
     "sptrsv_free_host": (None, [c_void_p]),
     "sptrsv_generate": (c_int, [c_int, c_int, c_uint64, _P_INT, _P_INT, _P_PINT, _P_PINT, _P_PDBL, _P_INT,
                                 _P_PDBL]),

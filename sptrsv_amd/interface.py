@@ -30,7 +30,8 @@ __all__ = [
     "generate_batch_member", "read_mtx_lower", "Batch", "RocsparseBaseline", "power_start",
     "power_end", "bench_solve", "device_name", "synchronize", "flush_cache", "Timer",
     "kernel_timer", "kernel_timer_read", "dep_latency_ns", "set_tolerance", "dominance_margin", "solve_graph",
-    "copy_device", "allocate_memory_upper", "transpose", "matrix_arrays",
+    "copy_device", "allocate_memory_upper", "transpose", "matrix_arrays", "read_mtx_coo", "coo_to_lower",
+    "save_csr", "load_csr",
 ]
 
 
@@ -354,6 +355,43 @@ def read_mtx_lower(path: str, unit: bool = True) -> tuple[int, np.ndarray, np.nd
     a = ctypes.POINTER(ctypes.c_double)()
     check(L.lib().sptrsv_read_mtx_lower(str(path).encode(), 1 if unit else 0, byref(n), byref(nnz), byref(ia),
                                         byref(ja), byref(a)), "sptrsv_read_mtx_lower")
+    return n.value, _take_int(ia, n.value + 1), _take_int(ja, nnz.value), _take_dbl(a, nnz.value)
+
+
+def read_mtx_coo(path: str, nthreads: int = 0) -> tuple[int, np.ndarray, np.ndarray, np.ndarray, bool]:
+    """main.cu:131-155 read over line-aligned chunks: (n, row, col, val, symmetric), 0-based, file order."""
+    n, nz, sym = c_int(), c_int(), c_int()
+    r, c = ctypes.POINTER(c_int)(), ctypes.POINTER(c_int)()
+    v = ctypes.POINTER(ctypes.c_double)()
+    check(L.lib().sptrsv_read_mtx_coo(str(path).encode(), nthreads, byref(n), byref(nz), byref(sym), byref(r),
+                                      byref(c), byref(v)), "sptrsv_read_mtx_coo")
+    return n.value, _take_int(r, nz.value), _take_int(c, nz.value), _take_dbl(v, nz.value), bool(sym.value)
+
+
+def coo_to_lower(row, col, val, n: int, symmetric: bool, unit: bool = True) -> SpMat:
+    """main.cu:124-246 on the device: COO -> A (mirrored if symmetric) -> lower triangle, as a handle."""
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    m = SpMat()
+    check(L.lib().sptrsv_coo_to_lower(_ptr(row), _ptr(col), _ptr(val), len(row), n, int(symmetric), int(unit),
+                                      byref(m)), "sptrsv_coo_to_lower")
+    return m
+
+
+def save_csr(path: str, ia, ja, a) -> None:
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja, np.int32)
+    a = np.ascontiguousarray(a, np.float64)
+    check(L.lib().sptrsv_save_csr(str(path).encode(), len(ia) - 1, _ptr(ia), _ptr(ja), _ptr(a)), "sptrsv_save_csr")
+
+
+def load_csr(path: str) -> tuple[int, np.ndarray, np.ndarray, np.ndarray]:
+    n, nnz = c_int(), c_int()
+    ia, ja = ctypes.POINTER(c_int)(), ctypes.POINTER(c_int)()
+    a = ctypes.POINTER(ctypes.c_double)()
+    check(L.lib().sptrsv_load_csr(str(path).encode(), byref(n), byref(nnz), byref(ia), byref(ja), byref(a)),
+          "sptrsv_load_csr")
     return n.value, _take_int(ia, n.value + 1), _take_int(ja, nnz.value), _take_dbl(a, nnz.value)
 
 

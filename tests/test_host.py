@@ -177,3 +177,117 @@ def test_generator_batch_members():
     assert 1 <= bw <= 32 and np.all(deg[bw:] == bw)      # dense band of width bw
     _, nl = O.levels(b.ia, b.ja)
     assert nl == b.n                                       # chain depth n
+
+
+# ---- parallel COO reader and the binary CSR cache ---------------------------------
+
+def write_mtx(path, n, ri, ci, v, sym=False, pattern=False, noise=False, crlf=False, seed=0):
+    """A coordinate file of 1-based entries, optionally with comment and blank
+    lines between entries and CRLF line ends (all skipped by the format)."""
+    rng = np.random.default_rng(seed)
+    field = "pattern" if pattern else "real"
+    lines = [f"%%MatrixMarket matrix coordinate {field} {'symmetric' if sym else 'general'}",
+             "% generated", f"{n} {n} {len(ri)}"]
+    for r, c, x in zip(ri, ci, v):
+        if noise and rng.random() < 0.01:
+            lines.append("% a comment between entries" if rng.random() < 0.5 else "")
+        lines.append(f"{r + 1} {c + 1}" if pattern else f"{r + 1} {c + 1} {float(x)!r}")
+    end = "\r\n" if crlf else "\n"
+    path.write_text(end.join(lines) + end)
+
+
+def random_coo(n, nz, rng, sym=False):
+    ri = rng.integers(0, n, nz).astype(np.int32)
+    ci = rng.integers(0, n, nz).astype(np.int32)
+    if sym:  # symmetric files hold one triangle
+        ri, ci = np.maximum(ri, ci), np.minimum(ri, ci)
+    return ri, ci, rng.uniform(-2, 2, nz)
+
+
+@pytest.mark.parametrize("name", ["tiny_general", "tiny_symmetric", "tiny_pattern"])
+@pytest.mark.parametrize("threads", [1, 3, 16])
+def test_mtx_coo_reader_fixtures(name, threads, golden):
+    path = golden["dir"] / f"{name}.mtx"
+    n, r, c, v, sym = I.read_mtx_coo(str(path), threads)
+    m, ri, ci, vi, s = O.read_mtx_coo(path)
+    assert (n, sym) == (m, s)
+    assert np.array_equal(r, ri) and np.array_equal(c, ci) and np.array_equal(v, vi)
+
+
+@pytest.mark.parametrize("sym,pattern,crlf", [(False, False, False), (True, False, True), (False, True, False)])
+def test_mtx_coo_reader_threads_agree(tmp_path, sym, pattern, crlf):
+    rng = np.random.default_rng(7)
+    n, nz = 3000, 60000          # ~1.3 MB: many line-aligned chunks
+    ri, ci, v = random_coo(n, nz, rng, sym)
+    path = tmp_path / "m.mtx"
+    write_mtx(path, n, ri, ci, v, sym=sym, pattern=pattern, noise=True, crlf=crlf)
+    want = v if not pattern else np.ones(nz)
+    for threads in (1, 2, 7, 16):
+        m, r, c, x, s = I.read_mtx_coo(str(path), threads)
+        assert (m, s) == (n, sym)
+        assert np.array_equal(r, ri) and np.array_equal(c, ci) and np.array_equal(x, want), threads
+    # the lower triangle built from the parallel read matches the oracle's pipeline
+    n2, ia, ja, a = I.read_mtx_lower(str(path), unit=False)
+    rp, col, val = O.coo_to_csr(n, ri, ci, want, sym)
+    lp, lc, lv = O.extract_lower(n, rp, col, val, unit=False)
+    assert np.array_equal(ia, lp) and np.array_equal(ja, lc) and np.array_equal(a, lv)
+
+
+def test_mtx_coo_reader_errors_in_later_chunks(tmp_path):
+    rng = np.random.default_rng(2)
+    n, nz = 2000, 40000
+    ri, ci, v = random_coo(n, nz, rng)
+    good = tmp_path / "g.mtx"
+    write_mtx(good, n, ri, ci, v)
+    text = good.read_text().splitlines()
+    # a malformed line near the end, before the last entry: premature end
+    bad = tmp_path / "b.mtx"
+    bad.write_text("\n".join(text[:-10] + ["7 x 1.0"] + text[-10:]) + "\n")
+    for threads in (1, 8):
+        with pytest.raises(L.SptrsvError) as e:
+            I.read_mtx_coo(str(bad), threads)
+        assert e.value.status == L.E_IO and "premature" in L.last_error()
+    # the same line after the last counted entry is never read
+    after = tmp_path / "a.mtx"
+    after.write_text("\n".join(text + ["7 x 1.0", "999999 1 1.0"]) + "\n")
+    for threads in (1, 8):
+        assert np.array_equal(I.read_mtx_coo(str(after), threads)[1], ri)
+    # an index out of range deep in the file names its entry
+    oob = tmp_path / "o.mtx"
+    oob.write_text("\n".join(text[:-5] + [f"{n + 1} 1 1.0"] + text[-4:]) + "\n")
+    for threads in (1, 8):
+        with pytest.raises(L.SptrsvError):
+            I.read_mtx_coo(str(oob), threads)
+        assert f"entry {nz - 4}" in L.last_error()
+    # fewer entries than announced
+    short = tmp_path / "s.mtx"
+    short.write_text("\n".join(text[:-3]) + "\n")
+    with pytest.raises(L.SptrsvError):
+        I.read_mtx_coo(str(short), 8)
+
+
+def test_csr_cache_round_trip_and_corruption(tmp_path):
+    s = I.generate(2, 20000)
+    path = tmp_path / "m.csr"
+    I.save_csr(str(path), s.ia, s.ja, s.a)
+    n, ia, ja, a = I.load_csr(str(path))
+    assert n == s.n and np.array_equal(ia, s.ia) and np.array_equal(ja, s.ja) and np.array_equal(a, s.a)
+    raw = bytearray(path.read_bytes())
+    raw[-3] ^= 0x40                       # one flipped bit in the values
+    bad = tmp_path / "bad.csr"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(L.SptrsvError) as e:
+        I.load_csr(str(bad))
+    assert e.value.status == L.E_IO and "checksum" in L.last_error()
+    cut = tmp_path / "cut.csr"
+    cut.write_bytes(bytes(raw[:-100]))
+    with pytest.raises(L.SptrsvError):
+        I.load_csr(str(cut))
+    notcsr = tmp_path / "x.csr"
+    notcsr.write_bytes(b"%%MatrixMarket matrix coordinate real general\n")
+    with pytest.raises(L.SptrsvError):
+        I.load_csr(str(notcsr))
+    empty = tmp_path / "e.csr"
+    I.save_csr(str(empty), np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    n, ia, ja, a = I.load_csr(str(empty))
+    assert n == 0 and ia.tolist() == [0] and len(ja) == 0
