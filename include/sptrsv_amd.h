@@ -122,6 +122,32 @@ int allocate_memory_upper(const int* csrRowPtrU, const int* csrColIdxU, const do
  * preconditioner from the L already uploaded.  dst is independent of src (its
  * own arrays; analyse it and free it like any other handle). */
 int sptrsv_transpose(sp_mat_t* src, sp_mat_t* dst);
+
+/* ---- spsv-style handle (no counterpart in the reference) --------------
+ * The create / analysis / solve / destroy sequence of the vendor sparse
+ * libraries' SpSV, over a device CSR (0-based, columns in any order).  Only
+ * the triangle named by fill is read; DIAG_UNIT ignores stored diagonal
+ * entries, DIAG_NON_UNIT uses each row's first (i, i) entry and rejects a
+ * row without one (SPTRSV_E_INVALID_CSR).  op(A) is A or A^T.  The handle
+ * owns device copies; the caller's arrays are not referenced afterwards. */
+#define SPTRSV_FILL_LOWER 0
+#define SPTRSV_FILL_UPPER 1
+#define SPTRSV_DIAG_NON_UNIT 0
+#define SPTRSV_DIAG_UNIT 1
+#define SPTRSV_OP_NONE 0
+#define SPTRSV_OP_TRANSPOSE 1
+typedef struct sptrsv_spsv sptrsv_spsv_t;
+int sptrsv_spsv_create(sptrsv_spsv_t** out, int n, int nnz, const int* d_ia, const int* d_ja,
+                       const double* d_a, int fill, int diag, int op);
+/* Level analysis; tune != 0 also autotunes the kernel for (precision, k). */
+int sptrsv_spsv_analysis(sptrsv_spsv_t* h, int precision, int k, int tune);
+/* op(A) x = alpha * b for k right-hand sides (device b, x; float* for
+ * SPTRSV_PREC_F32).  Stream-ordered, like sptrsv_solve. */
+int sptrsv_spsv_solve(sptrsv_spsv_t* h, int precision, double alpha, const void* b, void* x, int k,
+                      hipStream_t stream);
+/* The handle's matrix, for the sp_mat_t entry points (sptrsv_check, ...). */
+sp_mat_t* sptrsv_spsv_matrix(sptrsv_spsv_t* h);
+void sptrsv_spsv_destroy(sptrsv_spsv_t* h);
 /* interface.h:7 / interface.cu:28-38.  Runs the analysis for info->mode
  * (set by the caller) into the caller-allocated *info.  Returns
  * SPTRSV_E_NO_ANALYSIS_NEEDED (-1) for SIMPLE, like the reference. */

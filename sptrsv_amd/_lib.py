@@ -79,6 +79,12 @@ _PROTOS = {
     "allocate_memory_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
     "allocate_memory_upper": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
     "sptrsv_transpose": (c_int, [POINTER(SpMat), POINTER(SpMat)]),
+    "sptrsv_spsv_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                   c_int]),
+    "sptrsv_spsv_analysis": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "sptrsv_spsv_solve": (c_int, [c_void_p, c_int, ctypes.c_double, c_void_p, c_void_p, c_int, c_void_p]),
+    "sptrsv_spsv_matrix": (POINTER(SpMat), [c_void_p]),
+    "sptrsv_spsv_destroy": (None, [c_void_p]),
     "run_analysis": (c_int, [POINTER(AnalysisInfo), POINTER(SpMat)]),
     "run_solver": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int]),
     "free_device_analysis": (None, [POINTER(SpMat), POINTER(AnalysisInfo)]),

@@ -41,7 +41,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}",
 
 HIP_SOURCES = ["inst_f64.hip", "inst_f32.hip", "inst_mixed.hip", "inst_level.hip",
                "api_solve.hip", "analysis.hip", "interface.hip", "batch.hip", "iterative.hip",
-               "transpose.hip", "ingest.hip"]
+               "transpose.hip", "ingest.hip", "spsv.hip"]
 HOST_HIP_SOURCES = ["power.cpp", "rocsparse_baseline.cpp"]   # need HIP headers, no kernels
 CXX_SOURCES = ["mmio.cpp", "generate.cpp"]
 

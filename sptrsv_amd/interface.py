@@ -31,7 +31,8 @@ __all__ = [
     "power_end", "bench_solve", "device_name", "synchronize", "flush_cache", "Timer",
     "kernel_timer", "kernel_timer_read", "dep_latency_ns", "set_tolerance", "dominance_margin", "solve_graph",
     "copy_device", "allocate_memory_upper", "transpose", "matrix_arrays", "read_mtx_coo", "coo_to_lower",
-    "save_csr", "load_csr",
+    "save_csr", "load_csr", "Spsv", "FILL_LOWER", "FILL_UPPER", "DIAG_NON_UNIT", "DIAG_UNIT", "OP_NONE",
+    "OP_TRANSPOSE",
 ]
 
 
@@ -393,6 +394,44 @@ def load_csr(path: str) -> tuple[int, np.ndarray, np.ndarray, np.ndarray]:
     check(L.lib().sptrsv_load_csr(str(path).encode(), byref(n), byref(nnz), byref(ia), byref(ja), byref(a)),
           "sptrsv_load_csr")
     return n.value, _take_int(ia, n.value + 1), _take_int(ja, nnz.value), _take_dbl(a, nnz.value)
+
+
+FILL_LOWER, FILL_UPPER = 0, 1
+DIAG_NON_UNIT, DIAG_UNIT = 0, 1
+OP_NONE, OP_TRANSPOSE = 0, 1
+
+
+class Spsv:
+    """The spsv-style handle (sptrsv_spsv_*) over a device CSR: op(A) x = alpha b."""
+
+    def __init__(self, n: int, ia: DeviceArray, ja: DeviceArray, a: DeviceArray, fill: int = FILL_LOWER,
+                 diag: int = DIAG_NON_UNIT, op: int = OP_NONE):
+        self.n = n
+        h = c_void_p()
+        check(L.lib().sptrsv_spsv_create(byref(h), n, ja.count, ia.ptr, ja.ptr, a.ptr, fill, diag, op),
+              "sptrsv_spsv_create")
+        self.h = h
+
+    def analysis(self, precision: int = PREC_F64, k: int = 1, tune: bool = False) -> None:
+        check(L.lib().sptrsv_spsv_analysis(self.h, precision, k, int(tune)), "sptrsv_spsv_analysis")
+
+    def solve(self, b: DeviceArray, x: DeviceArray, precision: int = PREC_F64, alpha: float = 1.0, k: int = 1,
+              stream=None) -> None:
+        check(L.lib().sptrsv_spsv_solve(self.h, precision, alpha, b.ptr, x.ptr, k, stream), "sptrsv_spsv_solve")
+
+    def matrix(self) -> SpMat:
+        return L.lib().sptrsv_spsv_matrix(self.h).contents
+
+    def close(self) -> None:
+        if self.h is not None and self.h.value:
+            L.lib().sptrsv_spsv_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- batch, vendor baseline, power, timing ------------------------------------------
