@@ -122,6 +122,15 @@ int allocate_memory_upper(const int* csrRowPtrU, const int* csrColIdxU, const do
  * preconditioner from the L already uploaded.  dst is independent of src (its
  * own arrays; analyse it and free it like any other handle). */
 int sptrsv_transpose(sp_mat_t* src, sp_mat_t* dst);
+/* New values on the same sparsity pattern, keeping the handle's structure,
+ * its analyses and tuned kernel choices: no counterpart in the reference
+ * (whose analysis is redone per matrix).  vals is laid out like the value
+ * array the handle was created from (allocate_memory*, sptrsv_coo_to_lower,
+ * sptrsv_spsv_create; a transpose handle like its source's), count entries,
+ * in host memory or, with on_device != 0, device memory.  Stream-ordered
+ * after the device is idle; captured solves of the handle are dropped and
+ * re-captured on their next call. */
+int sptrsv_update_values(sp_mat_t* mat, const double* vals, long long count, int on_device);
 
 /* ---- spsv-style handle (no counterpart in the reference) --------------
  * The create / analysis / solve / destroy sequence of the vendor sparse

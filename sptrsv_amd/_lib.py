@@ -79,6 +79,7 @@ _PROTOS = {
     "allocate_memory_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
     "allocate_memory_upper": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(SpMat), c_int]),
     "sptrsv_transpose": (c_int, [POINTER(SpMat), POINTER(SpMat)]),
+    "sptrsv_update_values": (c_int, [POINTER(SpMat), c_void_p, c_int64, c_int]),
     "sptrsv_spsv_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                    c_int]),
     "sptrsv_spsv_analysis": (c_int, [c_void_p, c_int, c_int, c_int]),

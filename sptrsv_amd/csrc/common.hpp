@@ -77,6 +77,12 @@ struct MatPriv {
     // upper-triangular matrix stored as the lower triangle of the reversed
     // index order: solves reverse b into rev_b and the result back from rev_x
     bool reversed = false;
+    // stored entry q holds value src_perm[q] of the value array the handle
+    // was created from (src_count entries; identity when src_perm is null;
+    // -1 marks the constant 1.0 of a unit diagonal): sptrsv_update_values
+    // gathers new values through it, keeping structure and analysis
+    int* src_perm = nullptr;
+    long long src_count = 0;
     void* rev_b = nullptr;
     void* rev_x = nullptr;
     size_t rev_bytes = 0;

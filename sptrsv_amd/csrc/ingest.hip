@@ -74,39 +74,41 @@ __global__ void k_ing_emit(const int* __restrict__ row, const int* __restrict__ 
 }
 
 // Row i of A is order[rp[i] .. rp[i+1]) (emission indices in arrival order):
-// keep[i] = entries before the first column >= i, diag[i] = the diagonal
-// value; len[i] = keep[i] + 1 is the row's length in L.
+// keep[i] = entries before the first column >= i, dsrc[i] = the COO entry
+// giving the diagonal value (-1: the constant 1.0); len[i] = keep[i] + 1 is
+// the row's length in L.
 __global__ void k_ing_rows(const int* __restrict__ rp, const int* __restrict__ order, const int* __restrict__ ecol,
-                           const int* __restrict__ esrc, const double* __restrict__ val, int n, int unit,
-                           int* __restrict__ keep, double* __restrict__ diag, int* __restrict__ len) {
+                           const int* __restrict__ esrc, int n, int unit, int* __restrict__ keep,
+                           int* __restrict__ dsrc, int* __restrict__ len) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const int s = rp[i], e = rp[i + 1];
         int cut = e;
-        double d = 1.0;
+        int d = -1;
         bool have_d = unit != 0;
         for (int t = s; t < e; ++t) {
             const int g = order[t];
             const int c = ecol[g];
             if (c >= i && cut == e) cut = t;
             if (!have_d && c == i) {
-                d = val[esrc[g]];
+                d = esrc[g];
                 have_d = true;
             }
             if (cut != e && have_d) break;
         }
         keep[i] = cut - s;
-        diag[i] = d;
+        dsrc[i] = d;
         len[i] = cut - s + 1;
     }
 }
 
-// Writes L: kept entries of row i at ia[i] + local position, the diagonal last.
+// Writes L: kept entries of row i at ia[i] + local position, the diagonal
+// last; perm records each value's COO entry (-1: the constant 1.0).
 __global__ void k_ing_fill(const int* __restrict__ rp, const int* __restrict__ order,
                            const int* __restrict__ rows_sorted, const int* __restrict__ ecol,
                            const int* __restrict__ esrc, const double* __restrict__ val,
                            const int* __restrict__ keep, const int* __restrict__ ia, int total, int unit,
-                           int* __restrict__ ja, double* __restrict__ a) {
+                           int* __restrict__ ja, double* __restrict__ a, int* __restrict__ perm) {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
         const int i = rows_sorted[t];
@@ -115,16 +117,19 @@ __global__ void k_ing_fill(const int* __restrict__ rp, const int* __restrict__ o
             const int g = order[t];
             ja[ia[i] + local] = ecol[g];
             a[ia[i] + local] = unit ? 1.0 : val[esrc[g]];
+            perm[ia[i] + local] = unit ? -1 : esrc[g];
         }
     }
 }
 
-__global__ void k_ing_diag(const int* __restrict__ ia, const double* __restrict__ diag, int n, int* __restrict__ ja,
-                           double* __restrict__ a) {
+__global__ void k_ing_diag(const int* __restrict__ ia, const int* __restrict__ dsrc, const double* __restrict__ val,
+                           int n, int* __restrict__ ja, double* __restrict__ a, int* __restrict__ perm) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
-        ja[ia[i + 1] - 1] = (int)i;
-        a[ia[i + 1] - 1] = diag[i];
+        const int q = ia[i + 1] - 1;
+        ja[q] = (int)i;
+        a[q] = dsrc[i] >= 0 ? val[dsrc[i]] : 1.0;
+        perm[q] = dsrc[i];
     }
 }
 
@@ -154,8 +159,8 @@ int sptrsv_coo_to_lower(const int* row, const int* col, const double* val, int n
         if (*p) tmp.push_back(*p);
     };
     const size_t z = (size_t)nz;
-    int *d_row, *d_col, *emit, *pos, *bad, *cnt, *keep, *len;
-    double *d_val, *diag;
+    int *d_row, *d_col, *emit, *pos, *bad, *cnt, *keep, *len, *dsrc;
+    double* d_val;
     dalloc((void**)&d_row, 4 * z, "hipMalloc row");
     dalloc((void**)&d_col, 4 * z, "hipMalloc col");
     dalloc((void**)&d_val, 8 * z, "hipMalloc val");
@@ -165,7 +170,7 @@ int sptrsv_coo_to_lower(const int* row, const int* col, const double* val, int n
     dalloc((void**)&cnt, 4 * ((size_t)n + 1), "hipMalloc counts");
     dalloc((void**)&keep, 4 * ((size_t)n + 1), "hipMalloc keep");
     dalloc((void**)&len, 4 * ((size_t)n + 1), "hipMalloc lengths");
-    dalloc((void**)&diag, 8 * ((size_t)n + 1), "hipMalloc diag");
+    dalloc((void**)&dsrc, 4 * ((size_t)n + 1), "hipMalloc diagonal sources");
     if (st == SPTRSV_OK && nz > 0) {
         chk(hipMemcpy(d_row, row, 4 * z, hipMemcpyHostToDevice), "copy row");
         chk(hipMemcpy(d_col, col, 4 * z, hipMemcpyHostToDevice), "copy col");
@@ -203,8 +208,8 @@ int sptrsv_coo_to_lower(const int* row, const int* col, const double* val, int n
     if (st == SPTRSV_OK && n > 0) st = radix_sort_pairs(keys, vals, tk, tv, total, n - 1, nullptr);
     if (st == SPTRSV_OK) st = exclusive_scan(cnt, cnt, (long long)n + 1, nullptr);  // rows of A
     if (st == SPTRSV_OK && n > 0) {
-        hipLaunchKernelGGL(k_ing_rows, dim3(grid_for(n)), dim3(256), 0, nullptr, cnt, vals, ecol, esrc, d_val, n,
-                           unit, keep, diag, len);
+        hipLaunchKernelGGL(k_ing_rows, dim3(grid_for(n)), dim3(256), 0, nullptr, cnt, vals, ecol, esrc, n, unit, keep,
+                           dsrc, len);
         chk(hipGetLastError(), "ingest rows");
     }
     // row pointers of L
@@ -218,20 +223,23 @@ int sptrsv_coo_to_lower(const int* row, const int* col, const double* val, int n
     if (st == SPTRSV_OK) chk(hipMalloc(&gpu_L->ja, 4 * (size_t)std::max(nnzL, 1LL)), "hipMalloc ja");
     if (st == SPTRSV_OK) chk(hipMalloc(&gpu_L->a, 8 * (size_t)std::max(nnzL, 1LL)), "hipMalloc a");
     if (st == SPTRSV_OK) chk(hipMalloc(&gpu_L->a32, 4 * (size_t)std::max(nnzL, 1LL)), "hipMalloc a32");
+    MatPriv* pv = priv_of(gpu_L);
+    if (st == SPTRSV_OK) chk(hipMalloc(&pv->src_perm, 4 * (size_t)std::max(nnzL, 1LL)), "hipMalloc value map");
+    pv->src_count = nz;
     if (st == SPTRSV_OK && total > 0) {
         hipLaunchKernelGGL(k_ing_fill, dim3(grid_for(total)), dim3(256), 0, nullptr, cnt, vals, keys, ecol, esrc,
-                           d_val, keep, gpu_L->ia, total, unit, gpu_L->ja, gpu_L->a);
+                           d_val, keep, gpu_L->ia, total, unit, gpu_L->ja, gpu_L->a, pv->src_perm);
         chk(hipGetLastError(), "ingest fill");
     }
     if (st == SPTRSV_OK && n > 0) {
-        hipLaunchKernelGGL(k_ing_diag, dim3(grid_for(n)), dim3(256), 0, nullptr, gpu_L->ia, diag, n, gpu_L->ja,
-                           gpu_L->a);
+        hipLaunchKernelGGL(k_ing_diag, dim3(grid_for(n)), dim3(256), 0, nullptr, gpu_L->ia, dsrc, d_val, n,
+                           gpu_L->ja, gpu_L->a, pv->src_perm);
         chk(hipGetLastError(), "ingest diagonal");
     }
     if (st == SPTRSV_OK) {
         int mr = 0;
         for (int i = 0; i < n; ++i) mr = std::max(mr, ia_h[i + 1] - ia_h[i]);
-        priv_of(gpu_L)->max_row = mr;
+        pv->max_row = mr;
         chk(hipDeviceSynchronize(), "ingest sync");
     }
     if (st == SPTRSV_OK) st = convert_values_to_f32(gpu_L);

@@ -35,6 +35,21 @@ MatPriv* priv_of(sp_mat_t* m) { return static_cast<MatPriv*>(m->priv); }
 
 int convert_values_to_f32(sp_mat_t* m);
 void free_info_arrays(dfr_analysis_info_t* info);
+int jacobi_values_changed(sp_mat_t* m);
+
+// a[q] (and a32[q]) = src[perm[q]]; identity map when perm is null, the
+// constant 1.0 where perm[q] < 0.
+template <bool F64>
+__global__ void k_gather_values(const int* __restrict__ perm, const double* __restrict__ src, int nnz,
+                                double* __restrict__ a, float* __restrict__ a32) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int p = perm ? perm[q] : (int)q;
+        const double v = p >= 0 ? src[p] : 1.0;
+        if constexpr (F64) a[q] = v;
+        a32[q] = (float)v;
+    }
+}
 
 static double watchdog_ms_default() {
     const char* e = getenv("SPTRSV_WATCHDOG_MS");
@@ -78,6 +93,7 @@ void free_mat(sp_mat_t* m) {
         if (pv->corr) (void)hipFree(pv->corr);
         if (pv->resid) (void)hipFree(pv->resid);
         if (pv->trace) (void)hipFree(pv->trace);
+        if (pv->src_perm) (void)hipFree(pv->src_perm);
         if (pv->rev_b) (void)hipFree(pv->rev_b);
         if (pv->rev_x) (void)hipFree(pv->rev_x);
         for (float* p : {pv->jac_r, pv->jac_a, pv->jac_b, pv->jac_sval})
@@ -173,6 +189,7 @@ static int upload(const int* ia, const int* ja, const V* val, int n, sp_mat_t* m
         }
     }
     if (st == SPTRSV_OK) chk(hipDeviceSynchronize(), "upload sync");
+    if (st == SPTRSV_OK) priv_of(m)->src_count = nnz;
     if (st != SPTRSV_OK) free_mat(m);
     return st;
 }
@@ -332,7 +349,7 @@ int allocate_memory_upper(const int* csrRowPtrU, const int* csrColIdxU, const do
     if (base < 0) return fail(SPTRSV_E_INVALID_CSR, "negative row pointer");
     const long long nnz = (long long)csrRowPtrU[n] - base;
     if (nnz < n || nnz > 0x7fffffffLL) return fail(SPTRSV_E_INVALID_CSR, "bad row pointers");
-    std::vector<int> ia(n + 1), ja((size_t)nnz);
+    std::vector<int> ia(n + 1), ja((size_t)nnz), perm((size_t)nnz);
     std::vector<double> a((size_t)nnz);
     long long w = 0;
     ia[0] = 0;
@@ -350,18 +367,66 @@ int allocate_memory_upper(const int* csrRowPtrU, const int* csrColIdxU, const do
                                                       " outside the strict upper triangle");
             ja[(size_t)w] = n - 1 - j;
             a[(size_t)w] = csrValU[base + p];
+            perm[(size_t)w] = (int)p;
             ++w;
         }
         ia[r + 1] = (int)w;
     }
     SPT_TRY(upload(ia.data(), ja.data(), a.data(), n, gpu_U));
-    priv_of(gpu_U)->reversed = true;
-    return SPTRSV_OK;
+    MatPriv* pv = priv_of(gpu_U);
+    pv->reversed = true;
+    int st = SPTRSV_OK;
+    if (hipMalloc(&pv->src_perm, sizeof(int) * (size_t)nnz) != hipSuccess ||
+        hipMemcpy(pv->src_perm, perm.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess)
+        st = fail(SPTRSV_E_ALLOC, "value permutation");
+    if (st != SPTRSV_OK) free_mat(gpu_U);
+    return st;
 }
 
 void free_device_analysis(sp_mat_t* gpu_L, dfr_analysis_info_t* info) {
     if (info) free_info_arrays(info);
     if (gpu_L && gpu_L->priv) free_mat(gpu_L);
+}
+
+int sptrsv_update_values(sp_mat_t* mat, const double* vals, long long count, int on_device) {
+    if (!mat || !mat->priv) return fail(SPTRSV_E_INVALID_ARG, "matrix not allocated");
+    MatPriv* pv = priv_of(mat);
+    if (!pv->src_perm && pv->src_count != mat->nnz)
+        return fail(SPTRSV_E_INVALID_ARG, "this handle does not record where its values come from");
+    if (count != pv->src_count || (count > 0 && !vals))
+        return fail(SPTRSV_E_INVALID_ARG, "value count " + std::to_string(count) + " differs from the " +
+                                              std::to_string(pv->src_count) + " the handle was created from");
+    if (mat->nnz == 0) return SPTRSV_OK;
+    const double* src = vals;
+    double* staged = nullptr;
+    if (!on_device && count > 0) {
+        SPT_HIP(hipMalloc(&staged, sizeof(double) * (size_t)count));
+        if (hipMemcpy(staged, vals, sizeof(double) * (size_t)count, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(staged);
+            return fail(SPTRSV_E_HIP, "copy values");
+        }
+        src = staged;
+    }
+    const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(16384, ((long long)mat->nnz + 255) / 256));
+    if (mat->a)
+        hipLaunchKernelGGL(k_gather_values<true>, dim3(g), dim3(256), 0, nullptr, pv->src_perm, src, mat->nnz,
+                           mat->a, mat->a32);
+    else
+        hipLaunchKernelGGL(k_gather_values<false>, dim3(g), dim3(256), 0, nullptr, pv->src_perm, src, mat->nnz,
+                           nullptr, mat->a32);
+    int st = hipGetLastError() == hipSuccess ? SPTRSV_OK : fail(SPTRSV_E_HIP, "value gather launch");
+    if (st == SPTRSV_OK && hipDeviceSynchronize() != hipSuccess) st = fail(SPTRSV_E_HIP, "value gather");
+    if (staged) (void)hipFree(staged);
+    SPT_TRY(st);
+    // derived data: Jacobi's scaled copies and margin; captured solves bake
+    // the margin into their launches, so they are dropped; a tuned Jacobi
+    // choice may no longer apply (dominance can be lost)
+    if (mat->a) SPT_TRY(jacobi_values_changed(mat));
+    for (auto& gr : pv->graphs) (void)hipGraphExecDestroy(gr.second);
+    pv->graphs.clear();
+    for (int p = 0; p < 3; ++p)
+        if (pv->tuned[p] && pv->tuned_cfg[p].kernel == SPTRSV_KERNEL_JACOBI) pv->tuned[p] = false;
+    return SPTRSV_OK;
 }
 
 int run_solver(sp_mat_t* gpu_L, dfr_analysis_info_t* info, const double* b, double* x, int n) {

@@ -722,6 +722,27 @@ int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, 
     }
 }
 
+// New values on an unchanged structure: the row-scaled copy and the sorted
+// tile copies are rebuilt in their buffers (same addresses), and the
+// dominance margin is recomputed when next needed.
+int jacobi_values_changed(sp_mat_t* m) {
+    MatPriv* pv = priv_of(m);
+    pv->margin_known = false;
+    if (pv->jac_sval && m->nr > 0) {
+        const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(4096, ((long long)m->nr + 255) / 256));
+        hipLaunchKernelGGL(k_jac_scale, dim3(g), dim3(256), 0, nullptr, m->nr, m->ia, m->a, pv->jac_sval);
+        SPT_HIP(hipGetLastError());
+        for (TileTable& t : pv->jac_tt)
+            if (t.sja && t.count > 0) {
+                hipLaunchKernelGGL(k_jac_sort_tiles, dim3(t.count), dim3(256), 0, nullptr,
+                                   reinterpret_cast<const int4*>(t.tiles), m->ja, pv->jac_sval, t.sja, t.ssv, t.sslot);
+                SPT_HIP(hipGetLastError());
+            }
+        SPT_HIP(hipDeviceSynchronize());
+    }
+    return SPTRSV_OK;
+}
+
 // Smallest certified schedule for each sweeps-per-step candidate, timed; the
 // fastest is returned.  Certification is checked with a hashed right-hand
 // side and a safety factor of 8 on the tolerance.

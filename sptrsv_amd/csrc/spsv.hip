@@ -59,7 +59,7 @@ __global__ void k_tri_count(const int* __restrict__ ia, const int* __restrict__ 
 // (columns mapped to n-1-j for an upper triangle), then the diagonal.
 __global__ void k_tri_fill(const int* __restrict__ ia, const int* __restrict__ ja, const double* __restrict__ a,
                            int n, int upper, const int* __restrict__ dpos, const int* __restrict__ ia_s,
-                           int* __restrict__ ja_s, double* __restrict__ a_s) {
+                           int* __restrict__ ja_s, double* __restrict__ a_s, int* __restrict__ perm) {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (long long)gridDim.x * blockDim.x) {
         const int i = (int)t;
@@ -70,10 +70,12 @@ __global__ void k_tri_fill(const int* __restrict__ ia, const int* __restrict__ j
             if (j < 0 || j >= n || j == i || (upper ? j < i : j > i)) continue;
             ja_s[w] = upper ? n - 1 - j : j;
             a_s[w] = a[p];
+            perm[w] = p;
             ++w;
         }
         ja_s[w] = r;
         a_s[w] = dpos[i] >= 0 ? a[dpos[i]] : 1.0;
+        perm[w] = dpos[i];
     }
 }
 
@@ -116,13 +118,14 @@ int build_triangle(int n, int nnz, const int* ia, const int* ja, const double* a
     if (st == SPTRSV_OK) chk(hipMemcpy(ia_h.data(), m->ia, 4 * ia_h.size(), hipMemcpyDeviceToHost), "read ia");
     const int total = ia_h[n];
     m->nnz = total;
-    (void)nnz;
     if (st == SPTRSV_OK) chk(hipMalloc(&m->ja, 4 * (size_t)std::max(total, 1)), "hipMalloc ja");
     if (st == SPTRSV_OK) chk(hipMalloc(&m->a, 8 * (size_t)std::max(total, 1)), "hipMalloc a");
     if (st == SPTRSV_OK) chk(hipMalloc(&m->a32, 4 * (size_t)std::max(total, 1)), "hipMalloc a32");
+    if (st == SPTRSV_OK) chk(hipMalloc(&priv_of(m)->src_perm, 4 * (size_t)std::max(total, 1)), "hipMalloc value map");
+    priv_of(m)->src_count = nnz;
     if (st == SPTRSV_OK && n > 0) {
         hipLaunchKernelGGL(k_tri_fill, dim3(row_grid(n)), dim3(256), 0, nullptr, ia, ja, a, n, upper ? 1 : 0, dpos,
-                           m->ia, m->ja, m->a);
+                           m->ia, m->ja, m->a, priv_of(m)->src_perm);
         chk(hipGetLastError(), "triangle fill");
     }
     if (st == SPTRSV_OK) {

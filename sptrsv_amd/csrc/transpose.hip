@@ -58,6 +58,13 @@ __global__ void k_tr_fill(const int* __restrict__ ia, int n, int nnz, const int*
     }
 }
 
+// perm[q] = src_perm[perm[q]]: the value map of the source composed in.
+__global__ void k_tr_compose(int* __restrict__ perm, const int* __restrict__ src_perm, int nnz) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
+         q += (long long)gridDim.x * blockDim.x)
+        perm[q] = src_perm[perm[q]];
+}
+
 static unsigned tr_grid(long long count) {
     return (unsigned)std::max<long long>(1, std::min<long long>((count + 255) / 256, 16384));
 }
@@ -116,6 +123,17 @@ int sptrsv_transpose(sp_mat_t* src, sp_mat_t* dst) {
         for (int r = 0; r < n && st == SPTRSV_OK; ++r) mr = std::max(mr, ia[r + 1] - ia[r]);
         if (st == SPTRSV_OK && ia[n] != nnz) st = fail(SPTRSV_E_INVALID_CSR, "transpose row pointers do not sum to nnz");
         priv_of(dst)->max_row = mr;
+    }
+    // the sorted payload is the value map of the new handle
+    MatPriv* sp = priv_of(src);
+    if (st == SPTRSV_OK && nnz > 0 && sp->src_perm) {
+        hipLaunchKernelGGL(k_tr_compose, dim3(tr_grid(nnz)), dim3(256), 0, nullptr, vals, sp->src_perm, nnz);
+        chk(hipGetLastError(), "transpose value map");
+    }
+    if (st == SPTRSV_OK) {
+        priv_of(dst)->src_perm = vals;
+        priv_of(dst)->src_count = sp->src_count;
+        vals = nullptr;
     }
     for (int* p : {keys, vals, tk, tv})
         if (p) (void)hipFree(p);

@@ -31,7 +31,7 @@ __all__ = [
     "power_end", "bench_solve", "device_name", "synchronize", "flush_cache", "Timer",
     "kernel_timer", "kernel_timer_read", "dep_latency_ns", "set_tolerance", "dominance_margin", "solve_graph",
     "copy_device", "allocate_memory_upper", "transpose", "matrix_arrays", "read_mtx_coo", "coo_to_lower",
-    "save_csr", "load_csr", "Spsv", "FILL_LOWER", "FILL_UPPER", "DIAG_NON_UNIT", "DIAG_UNIT", "OP_NONE",
+    "save_csr", "load_csr", "update_values", "Spsv", "FILL_LOWER", "FILL_UPPER", "DIAG_NON_UNIT", "DIAG_UNIT", "OP_NONE",
     "OP_TRANSPOSE",
 ]
 
@@ -133,6 +133,15 @@ def transpose(mat: SpMat) -> SpMat:
     t = SpMat()
     check(L.lib().sptrsv_transpose(byref(mat), byref(t)), "sptrsv_transpose")
     return t
+
+
+def update_values(mat: SpMat, vals) -> None:
+    """New values (host numpy or DeviceArray) in the layout the handle was created from."""
+    if isinstance(vals, DeviceArray):
+        check(L.lib().sptrsv_update_values(byref(mat), vals.ptr, vals.count, 1), "sptrsv_update_values")
+    else:
+        v = np.ascontiguousarray(vals, np.float64)
+        check(L.lib().sptrsv_update_values(byref(mat), _ptr(v), len(v), 0), "sptrsv_update_values")
 
 
 def matrix_arrays(mat: SpMat) -> dict:
