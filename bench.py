@@ -60,25 +60,33 @@ def bytes_method_convention(n, nnz, k):
     return 2 * f32 + res + upd
 
 
-def bytes_jacobi_sweeps(n, nnz, k, sweeps, outer):
-    """Average algorithmic bytes of one Jacobi sweep launch over a solve.
+def _jacobi_sweep_totals(n, nnz, k, sweeps, outer, sweeps_next=0):
+    """(bytes, launches) of all sweep launches of a solve.
 
     A sweep reads ia, ja, a32 (strict lower part and diagonal), the previous
     iterate d (4kn, each entry once) and r32 (4kn).  Inner sweeps write d
     (4kn); the last sweep of a step writes x in FP64 (8kn) and, after the
-    first step, also reads it (8kn)."""
+    first step, also reads it (8kn).  Step 1 runs sweeps-1 launches, later
+    steps sweeps_next-1 (sweeps_next 0: the same as step 1)."""
+    per_step = [sweeps] + [sweeps_next or sweeps] * (outer - 1)
     base = 4 * (n + 1) + 8 * nnz + 8 * k * n
-    inner = (sweeps - 2) * outer * (base + 4 * k * n)
+    inner = sum(s - 2 for s in per_step) * (base + 4 * k * n)
     last = (base + 8 * k * n) + (outer - 1) * (base + 16 * k * n)
-    return (inner + last) / ((sweeps - 1) * outer)
+    return inner + last, sum(s - 1 for s in per_step)
 
 
-def bytes_jacobi_method(n, nnz, k, sweeps, outer):
+def bytes_jacobi_sweeps(n, nnz, k, sweeps, outer, sweeps_next=0):
+    """Average algorithmic bytes of one Jacobi sweep launch over a solve."""
+    total, launches = _jacobi_sweep_totals(n, nnz, k, sweeps, outer, sweeps_next)
+    return total / launches
+
+
+def bytes_jacobi_method(n, nnz, k, sweeps, outer, sweeps_next=0):
     """Whole Jacobi-refinement solve: per step one FP64 residual (the first
-    reads no x) and sweeps-1 sweeps, then the certifying FP64 residual."""
+    reads no x) and its sweeps, then the certifying FP64 residual."""
     res = 4 * (n + 1) + 12 * nnz + 16 * k * n + 4 * k * n + 4 * k * n   # b, x, r32, d0
     first_res = res - 8 * k * n
-    sweeps_b = bytes_jacobi_sweeps(n, nnz, k, sweeps, outer) * (sweeps - 1) * outer
+    sweeps_b, _ = _jacobi_sweep_totals(n, nnz, k, sweeps, outer, sweeps_next)
     cert = 4 * (n + 1) + 12 * nnz + 16 * k * n + 4 * k * n
     return first_res + (outer - 1) * res + sweeps_b + cert
 
@@ -90,7 +98,8 @@ def cfg_dict(c) -> dict:
     d = {"kernel": KERNEL_NAMES.get(c.kernel, c.kernel), "lanes": c.lanes, "wg": c.wg}
     if c.kernel == 4:
         d = {"kernel": "jacobi", "tile_products": 1024 * max(1, c.lanes), "wg": c.wg, "sweeps": c.sweeps,
-             "outer": c.outer, "entries_per_lane": c.window or 8, "column_sorted_tiles": bool(c.variant == 1)}
+             "sweeps_next": c.sweeps_next or c.sweeps, "outer": c.outer, "entries_per_lane": c.window or 8,
+             "column_sorted_tiles": bool(c.variant == 1)}
     else:
         d.update(fused=c.fused, variant={0: "multi-row-wave", 1: "one-row-per-wave", 2: "persistent"}.get(
             c.variant, c.variant), cu_waves=c.cu_waves, window=c.window)
@@ -145,7 +154,7 @@ def run_batch(args, I, O) -> int:
     kinds = [bt.choice(m).kernel for m in range(nmat)]
     jc = next((bt.choice(m) for m in range(nmat) if kinds[m] == I.KERNEL_JACOBI), None)
     log(f"[bench] tuned in {time.time() - t0:.1f}s: {kinds.count(I.KERNEL_JACOBI)} Jacobi members"
-        + (f" (sweeps {jc.sweeps}, steps {jc.outer})" if jc else ""))
+        + (f" (sweeps {jc.sweeps}/{jc.sweeps_next or jc.sweeps}, steps {jc.outer})" if jc else ""))
     for _ in range(args.warmup):
         bt.solve(I.PREC_MIXED, db, dx, graph=True)
     I.synchronize()
@@ -204,7 +213,8 @@ def run_batch(args, I, O) -> int:
         "config": {
             "workload": desc, "systems": nmat, "rows": rows, "nnz": nnz,
             "jacobi_members": kinds.count(I.KERNEL_JACOBI),
-            "jacobi_schedule": {"sweeps": jc.sweeps, "outer": jc.outer} if jc else None,
+            "jacobi_schedule": {"sweeps": jc.sweeps, "sweeps_next": jc.sweeps_next or jc.sweeps,
+                                "outer": jc.outer} if jc else None,
             "jacobi_nnz_fraction": round(bytes_jac_rows / nnz, 4),
             "rel_err_mixed_max_member": max(errs),
             "f64_batched_one_launch_ms": round(ms64b, 3),
@@ -233,10 +243,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--jacobi", type=int, nargs=5, metavar=("SWEEPS", "STEPS", "TILE", "EPL", "SORTED"),
-                    help="use the Jacobi-refinement mixed path with this schedule (tile width 1 or 2 = "
-                         "1024 or 2048 products, phase-2 entries per lane, 1 = column-sorted tiles) "
-                         "instead of tuning it")
+    ap.add_argument("--jacobi", type=int, nargs="+", metavar="N",
+                    help="SWEEPS STEPS TILE EPL SORTED [SWEEPS_NEXT]: use the Jacobi-refinement mixed path "
+                         "with this schedule (tile width 1 or 2 = 1024 or 2048 products, phase-2 entries per "
+                         "lane, 1 = column-sorted tiles, sweeps of the steps after the first) instead of "
+                         "tuning it")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the rocSPARSE, power and cold-cache legs (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -275,8 +286,11 @@ def main() -> int:
     tuned = {}
     for prec in (I.PREC_F64, I.PREC_MIXED):
         if prec == I.PREC_MIXED and args.jacobi:
-            tuned[prec] = I._cfg(I.KERNEL_JACOBI, args.jacobi[2], 256, 1, args.jacobi[4], 0, args.jacobi[0],
-                                 args.jacobi[1], args.jacobi[3])
+            j = args.jacobi
+            if len(j) not in (5, 6):
+                ap.error("--jacobi takes 5 or 6 integers")
+            tuned[prec] = I._cfg(I.KERNEL_JACOBI, j[2], 256, 1, j[4], 0, j[0], j[1], j[3],
+                                 j[5] if len(j) == 6 else 0)
         elif args.no_tune:
             tuned[prec] = I.default_config(mat, info, prec, k)
         else:
@@ -418,8 +432,8 @@ def main() -> int:
     cm = tuned[I.PREC_MIXED]
     traffic = None
     if cm.kernel == I.KERNEL_JACOBI:
-        b_launch = bytes_jacobi_sweeps(n, nnz, k, cm.sweeps, cm.outer)
-        b_method = bytes_jacobi_method(n, nnz, k, cm.sweeps, cm.outer)
+        b_launch = bytes_jacobi_sweeps(n, nnz, k, cm.sweeps, cm.outer, cm.sweeps_next)
+        b_method = bytes_jacobi_method(n, nnz, k, cm.sweeps, cm.outer, cm.sweeps_next)
         kpow = 1
         while kpow < k:
             kpow *= 2

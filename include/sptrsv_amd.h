@@ -245,6 +245,9 @@ typedef struct {
     int window;   /* sync-free: entries per lane kept in flight, 1, 4 or 16
                      (16: one right-hand side, wg 256); 0 = by longest row.
                      JACOBI: entries of a row one lane sums (0 = 8) */
+    int sweeps_next; /* JACOBI: sweeps of each refinement step after the
+                        first (0 = sweeps): later steps solve for a small
+                        correction and need less relative accuracy */
 } sptrsv_config_t;
 
 /* Relative accuracy the JACOBI path certifies (default 1e-10). */

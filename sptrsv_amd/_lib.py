@@ -59,7 +59,8 @@ class AnalysisInfo(Structure):  # dfr_analysis_info_t, dfr_syncfree.h:25-41
 
 class Config(Structure):  # sptrsv_config_t
     _fields_ = [("kernel", c_int), ("lanes", c_int), ("wg", c_int), ("fused", c_int), ("variant", c_int),
-                ("cu_waves", c_int), ("sweeps", c_int), ("outer", c_int), ("window", c_int)]
+                ("cu_waves", c_int), ("sweeps", c_int), ("outer", c_int), ("window", c_int),
+                ("sweeps_next", c_int)]
 
 
 class BenchResult(Structure):  # sptrsv_bench_t

@@ -176,13 +176,18 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
         // schedule not given: the tuned one, else 12 sweeps x 4 refinement steps
         const sptrsv_config_t& t = pv->tuned_cfg[precision];
         const bool tj = pv->tuned[precision] && pv->tuned_k[precision] == k && t.kernel == SPTRSV_KERNEL_JACOBI;
-        if (out->sweeps <= 0) out->sweeps = tj ? t.sweeps : 12;
+        if (out->sweeps <= 0) {
+            out->sweeps = tj ? t.sweeps : 12;
+            if (out->sweeps_next <= 0) out->sweeps_next = tj ? t.sweeps_next : 0;
+        }
         if (out->outer <= 0) out->outer = tj ? t.outer : 4;
     }
+    if (out->sweeps_next < 0) out->sweeps_next = 0;
     if (out->wg != 64 && out->wg != 256 && out->wg != 1024) out->wg = 256;
     if (K > 1) {
         out->wg = 256;
-        if (out->kernel != SPTRSV_KERNEL_JACOBI) out->variant = 0;  // no lean path with several RHS
+        // several RHS: the general engine, ticketed or persistent
+        if (out->kernel != SPTRSV_KERNEL_JACOBI && out->variant != 2) out->variant = 0;
     }
     if (out->cu_waves < 0) out->cu_waves = 0;
     if (out->kernel == SPTRSV_KERNEL_JACOBI) {
@@ -319,7 +324,7 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     int kind = c.kernel == SPTRSV_KERNEL_ORDER ? KK_ORDER
                : c.kernel == SPTRSV_KERNEL_MULTIROW ? KK_MULTIROW : KK_SIMPLE;
     if (c.variant == 1 && K == 1) kind += KK_FAST_SIMPLE - KK_SIMPLE;
-    if (c.variant == 2 && K == 1) {
+    if (c.variant == 2) {
         kind += KK_PERSIST_SIMPLE - KK_SIMPLE;
         // persistent: the wave count itself is the residency cap
         A.dyn_lds = pv->lds_pad;
@@ -489,9 +494,9 @@ int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, 
     sptrsv_config_t c;
     SPT_TRY(resolve_config(mat, info, precision, k, cfg, &c));
     MatPriv* pv = priv_of(mat);
-    const std::array<long long, 15> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
+    const std::array<long long, 16> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
                                         refine_steps, c.kernel, c.lanes, c.wg, c.fused, c.variant, c.cu_waves,
-                                        c.sweeps, c.outer, c.window};
+                                        c.sweeps, c.outer, c.window, c.sweeps_next};
     auto it = pv->graphs.find(key);
     if (it == pv->graphs.end()) {
         // one eager solve sets up every workspace and table (no allocation
@@ -680,7 +685,7 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
     // Shapes: (kernel, lanes, workgroup, variant).
     std::vector<sptrsv_config_t> shapes;
     int ref = std::min(reference_lanes(mat), kWave / K);
-    int lo = std::max(1, ref / 2), hi = std::min(kWave / K, ref * 4);
+    int lo = (K > 1) ? 1 : std::max(1, ref / 2), hi = std::min(kWave / K, ref * 4);
     std::vector<int> wgs = (K == 1) ? std::vector<int>{256, 1024} : std::vector<int>{256};
     // general engine (several rows per wave)
     for (int L = lo; L <= hi; L <<= 1)
@@ -689,6 +694,14 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
         for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, 256, 1, 0, 0});
     if (has_waves(info))
         for (int wg : wgs) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1, 0, 0});
+    // several right-hand sides: the persistent walk of the general engine
+    // (no ticket per workgroup; the wave count is the residency cap)
+    if (K > 1) {
+        for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, 256, 1, 2, 0});
+        if (has_order(info))
+            for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, 256, 1, 2, 0});
+        if (has_waves(info)) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, 256, 1, 2, 0});
+    }
     // lean path (one right-hand side): one row per wave, or single-level waves
     if (K == 1) {
         const int flo = std::max(4, ref), fhi = std::min(kWave, ref * 8);

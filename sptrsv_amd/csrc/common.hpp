@@ -98,7 +98,7 @@ struct MatPriv {
     int jac_members = 1;         // independent systems sharing the certificate (batch)
     // captured solves (sptrsv_solve_graph): key = precision, b, x, n, k,
     // refinement steps and the resolved configuration
-    std::map<std::array<long long, 15>, hipGraphExec_t> graphs;
+    std::map<std::array<long long, 16>, hipGraphExec_t> graphs;
     hipStream_t cap_stream = nullptr;
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};

@@ -664,7 +664,8 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     if (t0.count == 0) return SPTRSV_OK;
     const unsigned tiles = (unsigned)t0.count;
     const unsigned first_grid = ceil_div((long long)m->nr * K, 256);
-    const int sweeps = std::max(2, c.sweeps);
+    const int sweeps_first = std::max(2, c.sweeps);
+    const int sweeps_later = c.sweeps_next > 0 ? std::max(2, c.sweeps_next) : sweeps_first;
     const int outer = std::max(1, c.outer);
     const int epl = c.window > 0 ? c.window : 8;  // entries a phase-2 lane sums
     float* rs = pv->jac_r;  // D^-1 r of the current step; also the first iterate
@@ -676,6 +677,7 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
                                (const double*)x, rs, none, epl);
         const float* cur = rs;
         float* nxt = pv->jac_a;
+        const int sweeps = o == 0 ? sweeps_first : sweeps_later;
         for (int sw = 1; sw < sweeps; ++sw) {
             const bool last = sw == sweeps - 1;
             const int mode = last ? (o == 0 ? 1 : 2) : 0;
@@ -823,6 +825,32 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
         if (st == SPTRSV_OK && ms < bt) {
             bt = ms;
             bc = c;
+        }
+    }
+    // fewer sweeps in the steps after the first: they solve for a correction
+    // that needs only the remaining relative accuracy; stop at the first
+    // count that no longer certifies tol/8
+    for (int s2 = bc.sweeps - 1; s2 >= 2 && bc.outer > 1 && bt < 1e30f && st == SPTRSV_OK; --s2) {
+        sptrsv_config_t c = bc;
+        c.sweeps_next = s2;
+        int zero = 0;
+        SPT_HIP(hipMemcpy(pv->counters + 3, &zero, sizeof(int), hipMemcpyHostToDevice));
+        pv->tol = tol / 8;
+        st = jacobi_solve(m, c, k, b, x, nullptr);
+        pv->tol = tol;
+        int flag = 0;
+        SPT_HIP(hipMemcpy(&flag, pv->counters + 3, sizeof(int), hipMemcpyDeviceToHost));
+        if (st != SPTRSV_OK || flag != 0) break;
+        SPT_HIP(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
+        SPT_HIP(hipEventRecord(e1, nullptr));
+        SPT_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= std::max(reps, 1);
+        if (st == SPTRSV_OK && ms < bt) {
+            bt = ms;
+            bc.sweeps_next = s2;
         }
     }
     int zero = 0;

@@ -768,6 +768,45 @@ __global__ __launch_bounds__(WG) void k_persist(SolveArgs A) {
     }
 }
 
+// Persistent general engine (any K): the same resident-wave walk for lane
+// groups of L entry lanes x K right-hand-side lanes.  An item is the 64/(L*K)
+// rows one wave holds, consecutive in natural order (MODE 0) or in iorder
+// (MODE 1), or one multirow wave of the analysis (MODE 2, rows of one level,
+// in passes of 64/(L*K) rows).  Rows of an item may depend on each other in
+// MODE 0/1; sf_rows publishes inside its polling loop, so that is safe.  The
+// deadlock argument is k_persist's: items are walked in increasing order and
+// every dependency of the lowest unfinished item is lower or inside it.
+template <class P, int K, int WG, int U, int MODE>
+__global__ __launch_bounds__(WG) void k_persist_rows(SolveArgs A) {
+    const int W = gridDim.x * (WG / kWave);
+    const int wave = blockIdx.x * (WG / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    if constexpr (MODE == 2) {
+        for (int t = wave; t < A.n_waves; t += W) {
+            const int vs = A.wsize[t];
+            int L = vs == 0 ? 1 : vs;
+            if (L * K > kWave) L = kWave / K;
+            const int b0 = A.wbase[t];
+            const int nrows = A.wbase[t + 1] - b0;
+            const int cap = kWave / (L * K);
+            const int seg = lane / (L * K);
+            for (int c0 = 0; c0 < nrows; c0 += cap) {
+                const int row = (c0 + seg < nrows) ? A.iorder[b0 + c0 + seg] : -1;
+                sf_rows<P, K, U>(A, L, lane, row, nullptr, 0, 0);
+            }
+        }
+    } else {
+        const int lgG = A.lgL + kLog2<K>();
+        const int rpw = kWave >> lgG;
+        const int items = (A.n + rpw - 1) / rpw;
+        for (int t = wave; t < items; t += W) {
+            const int pos = t * rpw + (lane >> lgG);
+            const int row = pos < A.n ? (MODE == 1 ? A.iorder[pos] : pos) : -1;
+            sf_rows<P, K, U>(A, A.L, lane, row, nullptr, 0, 0);
+        }
+    }
+}
+
 // Fill the published words with the unsolved pattern and reset the ticket.
 template <class T>
 __global__ void k_prepare(T* pub, size_t count, T unsolved, int* ticket) {
@@ -784,9 +823,9 @@ enum KernelKind {
     KK_FAST_SIMPLE = 4,    // one row per wave, natural order (K == 1)
     KK_FAST_ORDER = 5,     // one row per wave, level order (K == 1)
     KK_FAST_MULTIROW = 6,  // multirow waves, no in-wave dependency (K == 1)
-    KK_PERSIST_SIMPLE = 7,    // persistent lean path over rows (K == 1)
-    KK_PERSIST_ORDER = 8,     // persistent lean path over iorder (K == 1)
-    KK_PERSIST_MULTIROW = 9,  // persistent lean path over multirow waves (K == 1)
+    KK_PERSIST_SIMPLE = 7,    // persistent walk over rows (lean for K == 1)
+    KK_PERSIST_ORDER = 8,     // persistent walk over iorder (lean for K == 1)
+    KK_PERSIST_MULTIROW = 9,  // persistent walk over multirow waves (lean for K == 1)
 };
 
 // Cooperative launch of a persistent kernel with A.persist_grid workgroups,
@@ -794,6 +833,21 @@ enum KernelKind {
 template <class P, int WG, int U, int MODE>
 hipError_t launch_persist(const SolveArgs& A, hipStream_t s) {
     const void* f = reinterpret_cast<const void*>(&k_persist<P, WG, U, MODE>);
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, A.dyn_lds);
+    if (e != hipSuccess) return e;
+    const long long cap = (long long)per_cu * cus;
+    unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(A.persist_grid, cap));
+    SolveArgs a = A;
+    void* args[] = {&a};
+    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(WG), args, A.dyn_lds, s);
+}
+
+template <class P, int K, int WG, int U, int MODE>
+hipError_t launch_persist_rows(const SolveArgs& A, hipStream_t s) {
+    const void* f = reinterpret_cast<const void*>(&k_persist_rows<P, K, WG, U, MODE>);
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -836,6 +890,17 @@ hipError_t launch_k(int kind, const SolveArgs& A, hipStream_t s) {
             default:
                 break;
         }
+    } else {
+        switch (kind) {
+            case KK_PERSIST_SIMPLE:
+                return launch_persist_rows<P, K, WG, U, 0>(A, s);
+            case KK_PERSIST_ORDER:
+                return launch_persist_rows<P, K, WG, U, 1>(A, s);
+            case KK_PERSIST_MULTIROW:
+                return launch_persist_rows<P, K, WG, U, 2>(A, s);
+            default:
+                break;
+        }
     }
     switch (kind) {
         case KK_SIMPLE:
@@ -855,7 +920,8 @@ hipError_t launch_k(int kind, const SolveArgs& A, hipStream_t s) {
 
 // Launch one solve pass.  K: right-hand-side lanes (1..64, power of two);
 // with K == 1 the workgroup may be 64, 256 or 1024 threads and the entry
-// window 1 or 4 (16 with 256 threads); with K > 1 both are fixed at 256 and 1.
+// window 1 or 4 (16 with 256 threads); with K > 1 the workgroup is 256
+// threads and the window 1 or 4.
 template <class P, int MAXK>
 hipError_t launch_solve(int kind, int K, int wg, int u, const SolveArgs& A, hipStream_t s) {
     if (K == 1) {
@@ -867,13 +933,14 @@ hipError_t launch_solve(int kind, int K, int wg, int u, const SolveArgs& A, hipS
         return (u == 4) ? launch_k<P, 1, 256, 4>(kind, A, s) : launch_k<P, 1, 256, 1>(kind, A, s);
     }
     if constexpr (MAXK > 1) {
+        const bool w4 = u >= 4;
         switch (K) {
-            case 2: return launch_k<P, 2, 256, 1>(kind, A, s);
-            case 4: return launch_k<P, 4, 256, 1>(kind, A, s);
-            case 8: return launch_k<P, 8, 256, 1>(kind, A, s);
-            case 16: return launch_k<P, 16, 256, 1>(kind, A, s);
-            case 32: return launch_k<P, 32, 256, 1>(kind, A, s);
-            case 64: return launch_k<P, 64, 256, 1>(kind, A, s);
+            case 2: return w4 ? launch_k<P, 2, 256, 4>(kind, A, s) : launch_k<P, 2, 256, 1>(kind, A, s);
+            case 4: return w4 ? launch_k<P, 4, 256, 4>(kind, A, s) : launch_k<P, 4, 256, 1>(kind, A, s);
+            case 8: return w4 ? launch_k<P, 8, 256, 4>(kind, A, s) : launch_k<P, 8, 256, 1>(kind, A, s);
+            case 16: return w4 ? launch_k<P, 16, 256, 4>(kind, A, s) : launch_k<P, 16, 256, 1>(kind, A, s);
+            case 32: return w4 ? launch_k<P, 32, 256, 4>(kind, A, s) : launch_k<P, 32, 256, 1>(kind, A, s);
+            case 64: return w4 ? launch_k<P, 64, 256, 4>(kind, A, s) : launch_k<P, 64, 256, 1>(kind, A, s);
             default: break;
         }
     }

@@ -248,16 +248,17 @@ def csr_L_solve_mixed(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, ref
 
 
 def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0, sweeps=0, outer=0,
-         window=0) -> Config:
-    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window)
+         window=0, sweeps_next=0) -> Config:
+    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window, sweeps_next)
 
 
 def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
           kernel: int = KERNEL_AUTO, lanes: int = 0, wg: int = 0, fused: int = 1, variant: int = 0,
-          cu_waves: int = 0, sweeps: int = 0, outer: int = 0, window: int = 0, stream=None,
-          cfg: Config | None = None) -> None:
+          cu_waves: int = 0, sweeps: int = 0, outer: int = 0, window: int = 0, sweeps_next: int = 0,
+          stream=None, cfg: Config | None = None) -> None:
     """cfg, when given, is used as is (e.g. the result of autotune)."""
-    cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window)
+    cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window,
+                                           sweeps_next)
     check(L.lib().sptrsv_solve(byref(mat), byref(info) if info is not None else None, precision, byref(cfg),
                                b.ptr, x.ptr, mat.nr, k, refine_steps, stream), "sptrsv_solve")
 

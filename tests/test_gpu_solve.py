@@ -285,6 +285,26 @@ def test_multi_rhs(device):
     d.close()
 
 
+def test_multi_rhs_persistent_walk(device):
+    # several right-hand sides on the persistent walk (variant 2): natural
+    # order, level order and multirow waves, with and without residency caps
+    for cfg, param in ((4, 24), (2, 60000), (3, 60000)):
+        s = I.generate(cfg, param)
+        d = Dev(s.ia, s.ja, s.a)
+        for k in (3, 16):
+            B = np.random.default_rng(k + cfg).uniform(-1, 1, s.n * k)
+            ref = O.fwd_f64(s.ia, s.ja, s.a, B, k=k)
+            for name, kern in KERNELS.items():
+                for lanes in ((1, 2) if kern != I.KERNEL_MULTIROW else (0,)):
+                    for cw in (0, 8):
+                        print(cfg, k, name, lanes, cw, flush=True)
+                        x = d.solve(B, kernel=kern, k=k, lanes=lanes, wg=256, variant=2, cu_waves=cw)
+                        assert relerr(x, ref) <= 1e-12, (cfg, k, name, lanes, cw)
+                x = d.solve(B, precision=I.PREC_MIXED, kernel=kern, k=k, wg=256, variant=2, cu_waves=4)
+                assert relerr(x, ref) <= 1e-10, (cfg, k, name, "mixed")
+        d.close()
+
+
 def test_bit_reproducible(device):
     s = I.generate(3, 100000)
     d = Dev(s.ia, s.ja, s.a)
