@@ -99,7 +99,8 @@ def cfg_dict(c) -> dict:
     if c.kernel == 4:
         d = {"kernel": "jacobi", "tile_products": 1024 * max(1, c.lanes), "wg": c.wg, "sweeps": c.sweeps,
              "sweeps_next": c.sweeps_next or c.sweeps, "outer": c.outer, "entries_per_lane": c.window or 8,
-             "column_sorted_tiles": bool(c.variant == 1)}
+             "column_sorted_tiles": bool(c.variant == 1),
+             "block_inner_iterations": (c.inner or 4) if c.variant == 2 else 0}
     else:
         d.update(fused=c.fused, variant={0: "multi-row-wave", 1: "one-row-per-wave", 2: "persistent"}.get(
             c.variant, c.variant), cu_waves=c.cu_waves, window=c.window)
@@ -154,7 +155,8 @@ def run_batch(args, I, O) -> int:
     kinds = [bt.choice(m).kernel for m in range(nmat)]
     jc = next((bt.choice(m) for m in range(nmat) if kinds[m] == I.KERNEL_JACOBI), None)
     log(f"[bench] tuned in {time.time() - t0:.1f}s: {kinds.count(I.KERNEL_JACOBI)} Jacobi members"
-        + (f" (sweeps {jc.sweeps}/{jc.sweeps_next or jc.sweeps}, steps {jc.outer})" if jc else ""))
+        + (f" (sweeps {jc.sweeps}/{jc.sweeps_next or jc.sweeps}, steps {jc.outer}"
+           + (f", block sweeps with {jc.inner or 4} inner iterations" if jc.variant == 2 else "") + ")" if jc else ""))
     for _ in range(args.warmup):
         bt.solve(I.PREC_MIXED, db, dx, graph=True)
     I.synchronize()
@@ -214,7 +216,8 @@ def run_batch(args, I, O) -> int:
             "workload": desc, "systems": nmat, "rows": rows, "nnz": nnz,
             "jacobi_members": kinds.count(I.KERNEL_JACOBI),
             "jacobi_schedule": {"sweeps": jc.sweeps, "sweeps_next": jc.sweeps_next or jc.sweeps,
-                                "outer": jc.outer} if jc else None,
+                                "outer": jc.outer,
+                                "block_inner_iterations": (jc.inner or 4) if jc.variant == 2 else 0} if jc else None,
             "jacobi_nnz_fraction": round(bytes_jac_rows / nnz, 4),
             "rel_err_mixed_max_member": max(errs),
             "f64_batched_one_launch_ms": round(ms64b, 3),

@@ -248,6 +248,9 @@ typedef struct {
     int sweeps_next; /* JACOBI: sweeps of each refinement step after the
                         first (0 = sweeps): later steps solve for a small
                         correction and need less relative accuracy */
+    int inner;       /* JACOBI with variant 2 (block sweeps, one right-hand
+                        side): Jacobi iterations on each tile's diagonal
+                        block per sweep, in LDS (0 = 4) */
 } sptrsv_config_t;
 
 /* Relative accuracy the JACOBI path certifies (default 1e-10). */
@@ -332,6 +335,11 @@ int sptrsv_batch_solve_graph(sptrsv_batch_t* bt, int precision, const double* b,
                              double* x, hipStream_t stream);
 int sptrsv_batch_row_offsets(const sptrsv_batch_t* bt, int64_t* row_off /* nmat+1 */);
 int sptrsv_batch_choice(const sptrsv_batch_t* bt, int m, sptrsv_config_t* cfg);
+/* How a MIXED batched solve runs its two halves (set by sptrsv_batch_tune,
+ * or by the caller): serial 1 runs the Jacobi members, then the sync-free
+ * ones, on the caller's stream; 0 runs them concurrently on two streams. */
+int sptrsv_batch_schedule(const sptrsv_batch_t* bt, int* serial);
+int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial);
 /* sptrsv_check for the batch: watchdog, or a Jacobi member that missed its
  * certified bound (each member is certified separately). */
 int sptrsv_batch_check(sptrsv_batch_t* bt, hipStream_t stream);

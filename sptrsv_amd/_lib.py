@@ -60,7 +60,7 @@ class AnalysisInfo(Structure):  # dfr_analysis_info_t, dfr_syncfree.h:25-41
 class Config(Structure):  # sptrsv_config_t
     _fields_ = [("kernel", c_int), ("lanes", c_int), ("wg", c_int), ("fused", c_int), ("variant", c_int),
                 ("cu_waves", c_int), ("sweeps", c_int), ("outer", c_int), ("window", c_int),
-                ("sweeps_next", c_int)]
+                ("sweeps_next", c_int), ("inner", c_int)]
 
 
 class BenchResult(Structure):  # sptrsv_bench_t
@@ -118,6 +118,8 @@ _PROTOS = {
     "sptrsv_batch_row_offsets": (c_int, [c_void_p, POINTER(c_int64)]),
     "sptrsv_batch_choice": (c_int, [c_void_p, c_int, POINTER(Config)]),
     "sptrsv_batch_check": (c_int, [c_void_p, c_void_p]),
+    "sptrsv_batch_schedule": (c_int, [c_void_p, _P_INT]),
+    "sptrsv_batch_set_schedule": (c_int, [c_void_p, c_int]),
     "sptrsv_batch_destroy": (None, [c_void_p]),
     "csr_L_rocsparse_create": (c_int, [POINTER(SpMat), POINTER(c_void_p), POINTER(c_float)]),
     "csr_L_solve_rocsparse": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

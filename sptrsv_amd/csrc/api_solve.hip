@@ -179,10 +179,12 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
         if (out->sweeps <= 0) {
             out->sweeps = tj ? t.sweeps : 12;
             if (out->sweeps_next <= 0) out->sweeps_next = tj ? t.sweeps_next : 0;
+            if (out->inner <= 0) out->inner = tj ? t.inner : 0;
         }
         if (out->outer <= 0) out->outer = tj ? t.outer : 4;
     }
     if (out->sweeps_next < 0) out->sweeps_next = 0;
+    if (out->inner < 0 || out->inner > 64) return fail(SPTRSV_E_INVALID_ARG, "inner must be 0..64");
     if (out->wg != 64 && out->wg != 256 && out->wg != 1024) out->wg = 256;
     if (K > 1) {
         out->wg = 256;
@@ -494,9 +496,9 @@ int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, 
     sptrsv_config_t c;
     SPT_TRY(resolve_config(mat, info, precision, k, cfg, &c));
     MatPriv* pv = priv_of(mat);
-    const std::array<long long, 16> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
+    const std::array<long long, 17> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
                                         refine_steps, c.kernel, c.lanes, c.wg, c.fused, c.variant, c.cu_waves,
-                                        c.sweeps, c.outer, c.window, c.sweeps_next};
+                                        c.sweeps, c.outer, c.window, c.sweeps_next, c.inner};
     auto it = pv->graphs.find(key);
     if (it == pv->graphs.end()) {
         // one eager solve sets up every workspace and table (no allocation

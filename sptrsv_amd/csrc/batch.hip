@@ -94,6 +94,7 @@ struct sptrsv_batch {
     Task* d_tasks_sf = nullptr;          // MIXED: the sync-free matrices only
     int ntasks_sf = 0;
     hipStream_t aux = nullptr;           // Jacobi half of a MIXED solve
+    bool serial = false;                 // MIXED: Jacobi half, then the sync-free half, on one stream
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     Task* d_sub = nullptr;               // tuning: tasks of one matrix
     int sub_cap = 0;
@@ -231,6 +232,11 @@ int batch_solve_all(sptrsv_batch_t* bt, int precision, const double* b, double* 
         return batch_solve_tasks(bt, precision, bt->d_tasks, precision == SPTRSV_PREC_MIXED ? bt->ntasks_sf
                                                                                            : bt->ntasks,
                                  b, x, s);
+    if (bt->serial) {  // the two halves one after the other, each with the whole device
+        SPT_TRY(jacobi_solve(&bt->mat, bt->jac_cfg, 1, b, x, s));
+        if (bt->ntasks_sf > 0) SPT_TRY(batch_solve_tasks(bt, precision, bt->d_tasks_sf, bt->ntasks_sf, b, x, s));
+        return SPTRSV_OK;
+    }
     SPT_HIP(hipEventRecord(bt->fork_ev, s));
     SPT_HIP(hipStreamWaitEvent(bt->aux, bt->fork_ev, 0));
     SPT_TRY(jacobi_solve(&bt->mat, bt->jac_cfg, 1, b, x, bt->aux));
@@ -331,6 +337,20 @@ int sptrsv_batch_choice(const sptrsv_batch_t* bt, int m, sptrsv_config_t* cfg) {
     if (!bt || !cfg || m < 0 || m >= (int)bt->lanes.size()) return fail(SPTRSV_E_INVALID_ARG, "bad index");
     *cfg = sptrsv_config_t{SPTRSV_KERNEL_SIMPLE, bt->lanes[m], kBatchWG, 1};
     if (bt->choice[m] == SPTRSV_KERNEL_JACOBI) *cfg = bt->jac_cfg;
+    return SPTRSV_OK;
+}
+
+int sptrsv_batch_schedule(const sptrsv_batch_t* bt, int* serial) {
+    if (!bt || !serial) return fail(SPTRSV_E_INVALID_ARG, "null argument");
+    *serial = bt->serial ? 1 : 0;
+    return SPTRSV_OK;
+}
+
+int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial) {
+    if (!bt || serial < 0 || serial > 1) return fail(SPTRSV_E_INVALID_ARG, "bad batch schedule");
+    bt->serial = serial != 0;
+    for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
+    bt->graphs.clear();
     return SPTRSV_OK;
 }
 
@@ -458,6 +478,20 @@ int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
         }
     }
     SPT_TRY(apply_choice(bt, best_set, nullptr));
+    // the two halves concurrently on two streams, or one after the other
+    // (the concurrent Jacobi sweeps can starve the latency-bound sync-free
+    // waves)
+    bt->serial = false;
+    if (bt->n_jacobi > 0 && bt->ntasks_sf > 0) {
+        const float tc = time_batch(bt, ts.b, ts.x, 5, &st);
+        SPT_TRY(st);
+        bt->serial = true;
+        const float tsr = time_batch(bt, ts.b, ts.x, 5, &st);
+        SPT_TRY(st);
+        bt->serial = tsr < tc;
+    }
+    for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
+    bt->graphs.clear();
     return sptrsv_check(&bt->mat, nullptr);
 }
 

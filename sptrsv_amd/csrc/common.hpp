@@ -88,6 +88,7 @@ struct MatPriv {
     size_t rev_bytes = 0;
     float* jac_r = nullptr;      // FP32 row-scaled residual D^-1 r, n*k
     float* jac_sval = nullptr;   // FP32 a_ij / a_ii (0 on the diagonal), nnz
+    double* jac_diag = nullptr;  // a_ii, n (contiguous, for the first iterate)
     float* jac_a = nullptr;      // Jacobi iterates (ping-pong), n*k each
     float* jac_b = nullptr;
     size_t jac_len = 0;
@@ -98,7 +99,7 @@ struct MatPriv {
     int jac_members = 1;         // independent systems sharing the certificate (batch)
     // captured solves (sptrsv_solve_graph): key = precision, b, x, n, k,
     // refinement steps and the resolved configuration
-    std::map<std::array<long long, 16>, hipGraphExec_t> graphs;
+    std::map<std::array<long long, 17>, hipGraphExec_t> graphs;
     hipStream_t cap_stream = nullptr;
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};

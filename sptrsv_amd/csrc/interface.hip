@@ -98,6 +98,7 @@ void free_mat(sp_mat_t* m) {
         if (pv->rev_x) (void)hipFree(pv->rev_x);
         for (float* p : {pv->jac_r, pv->jac_a, pv->jac_b, pv->jac_sval})
             if (p) (void)hipFree(p);
+        if (pv->jac_diag) (void)hipFree(pv->jac_diag);
         if (pv->jac_norms) (void)hipFree(pv->jac_norms);
         for (auto& g : pv->graphs) (void)hipGraphExecDestroy(g.second);
         if (pv->cap_stream) (void)hipStreamDestroy(pv->cap_stream);

@@ -61,6 +61,7 @@ struct TileArgs {
     const int* ja;
     const float* sval;        // a_ij / a_ii in FP32 (0 on the diagonal)
     const double* a64;
+    const double* diag;       // a_ii, contiguous
     const int4* tiles;        // per tile {first row, end row, first entry, end entry}
     const int* tile_member;   // member (independent system) of each tile; null: one member
     // column-sorted tiles (sweep only): each tile's entries ordered by column,
@@ -272,6 +273,178 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const fl
     sweep_tile<K, TW, SORTED>(A, xcd_tile(blockIdx.x, gridDim.x), rs, d_old, d_new, x, mode, epl, prod, s_ia, s_r);
 }
 
+// Block sweep (one right-hand side): the coupling to earlier tiles is taken
+// from the previous iterate once, and the tile's own diagonal block gets
+// `inner` Jacobi iterations in LDS:
+//   base_i = rs_i - sum_{j < r0} N'_ij d_old_j
+//   z^0 = d_old on the tile, z^t_i = base_i - sum_{r0 <= j < i} N'_ij z^{t-1}_j
+//   d_new = z^inner
+// One inner iteration is the point sweep restricted to the tile; with as
+// many inner iterations as the tile has rows it is the exact block solve.
+// Banded and locally coupled systems need a fraction of the point sweeps,
+// and an inner iteration costs LDS traffic only.  A tile too large for LDS
+// takes the point update.
+template <int TW>
+__global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const float* __restrict__ rs,
+                                                             const float* __restrict__ d_old,
+                                                             float* __restrict__ d_new, double* x, int mode,
+                                                             int epl, int inner) {
+    constexpr int W = TW;                   // max rows of a tile (one entry per row at least)
+    constexpr int CAP = tile_cap<TW>();
+    constexpr int U = CAP / kTileThreads;
+    constexpr int RU = (W + 1 + kTileThreads - 1) / kTileThreads;
+    __shared__ float s_v[CAP];              // off-tile product, or in-tile value
+    __shared__ int s_c[CAP];                // -1 off-tile, else the tile-local column
+    __shared__ int s_ia[W + 1];
+    __shared__ float s_b[W];                // base
+    __shared__ float s_z[2][W];             // inner iterates
+    const int tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int4 tl = A.tiles[tile];
+    const int r0 = tl.x, rows = tl.y - tl.x, p0 = tl.z, E = tl.w - tl.z;
+    if (rows <= 0) return;
+    if (E > CAP) {  // point update straight from memory, a wavefront per row
+        const int lane = tid & (kWave - 1);
+        for (int lr = tid >> 6; lr < rows; lr += kTileThreads / kWave) {
+            const int i = r0 + lr;
+            const int q0 = A.ia[i], q1 = A.ia[i + 1] - 1;
+            float sm = 0.0f;
+            for (int q = q0 + lane; q < q1; q += kWave) sm = __fmaf_rn(A.sval[q], d_old[A.ja[q]], sm);
+            sm = jac_group_sum(sm, kWave, 1);
+            if (lane == 0) {
+                const float d = rs[i] - sm;
+                if (mode == 0) d_new[i] = d;
+                else if (mode == 1) x[i] = (double)d;
+                else x[i] += (double)d;
+            }
+        }
+        return;
+    }
+    // staging: every load issued before its first use
+    int iav[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) iav[u] = (u * kTileThreads <= rows) ? A.ia[r0 + min(tid + u * kTileThreads, rows)] : 0;
+    float rv[RU], zv[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+        const int lr = min(tid + u * kTileThreads, rows - 1);
+        rv[u] = (u * kTileThreads < rows) ? rs[r0 + lr] : 0.0f;
+        zv[u] = (u * kTileThreads < rows) ? d_old[r0 + lr] : 0.0f;
+    }
+    int jv[U];
+    float av[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int e = min(tid + u * kTileThreads, E - 1);
+        jv[u] = (u * kTileThreads < E) ? A.ja[p0 + e] : r0;
+        av[u] = (u * kTileThreads < E) ? A.sval[p0 + e] : 0.0f;
+    }
+    float xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = (jv[u] < r0) ? d_old[jv[u]] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+        const int t = tid + u * kTileThreads;
+        if (t <= rows) s_ia[t] = iav[u] - p0;
+        if (t < rows) {
+            s_b[t] = rv[u];
+            s_z[0][t] = zv[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int t = tid + u * kTileThreads;
+        if (t < E) {
+            const bool off = jv[u] < r0;
+            s_v[t] = off ? av[u] * xv[u] : av[u];
+            s_c[t] = off ? -1 : jv[u] - r0;
+        }
+    }
+    __syncthreads();
+    const int G2 = group_lanes<1>(E, rows, epl);
+    const int rpp = kTileThreads / G2;
+    const int npass = (rows + rpp - 1) / rpp;
+    const int e = tid % G2;
+    // base: the off-tile sums
+    for (int ps = 0; ps < npass; ++ps) {
+        const int lr = ps * rpp + tid / G2;
+        const bool live = lr < rows;
+        float sm = 0.0f;
+        if (live) {
+            const int q1 = s_ia[lr + 1] - 1;  // diagonal excluded
+            for (int q = s_ia[lr] + e; q < q1; q += G2)
+                if (s_c[q] < 0) sm += s_v[q];
+        }
+        sm = jac_group_sum(sm, G2, 1);
+        if (live && e == 0) s_b[lr] -= sm;
+    }
+    // inner iterations on the diagonal block.  With one pass of rows per
+    // workgroup (the usual case) each lane keeps its first EC in-tile
+    // entries in registers across the iterations, so an iteration costs one
+    // round of independent LDS gathers.
+    constexpr int EC = 8;
+    int cur = 0;
+    if (npass == 1) {
+        __syncthreads();
+        const int lr = tid / G2;
+        const bool live = lr < rows;
+        const int q0 = live ? s_ia[lr] + e : 0, q1 = live ? s_ia[lr + 1] - 1 : 0;
+        int cc[EC];
+        float cv[EC];
+#pragma unroll
+        for (int u = 0; u < EC; ++u) {
+            const int q = q0 + u * G2;
+            cc[u] = (q < q1) ? s_c[q] : -1;
+            cv[u] = (cc[u] >= 0) ? s_v[q] : 0.0f;
+        }
+        const int qrest = q0 + EC * G2;
+        for (int it = 0; it < inner; ++it) {
+            if (it > 0) __syncthreads();
+            const float* zc = s_z[cur];
+            float sm = 0.0f;
+#pragma unroll
+            for (int u = 0; u < EC; ++u) sm = __fmaf_rn(cv[u], zc[max(cc[u], 0)], sm);
+            for (int q = qrest; q < q1; q += G2) {
+                const int c = s_c[q];
+                if (c >= 0) sm = __fmaf_rn(s_v[q], zc[c], sm);
+            }
+            sm = jac_group_sum(sm, G2, 1);
+            if (live && e == 0) s_z[cur ^ 1][lr] = s_b[lr] - sm;
+            cur ^= 1;
+        }
+    } else {
+        for (int it = 0; it < inner; ++it) {
+            __syncthreads();
+            const float* zc = s_z[cur];
+            float* zn = s_z[cur ^ 1];
+            for (int ps = 0; ps < npass; ++ps) {
+                const int lr = ps * rpp + tid / G2;
+                const bool live = lr < rows;
+                float sm = 0.0f;
+                if (live) {
+                    const int q1 = s_ia[lr + 1] - 1;
+                    for (int q = s_ia[lr] + e; q < q1; q += G2) {
+                        const int c = s_c[q];
+                        if (c >= 0) sm = __fmaf_rn(s_v[q], zc[c], sm);
+                    }
+                }
+                sm = jac_group_sum(sm, G2, 1);
+                if (live && e == 0) zn[lr] = s_b[lr] - sm;
+            }
+            cur ^= 1;
+        }
+    }
+    __syncthreads();
+    const float* zf = s_z[cur];
+    for (int lr = tid; lr < rows; lr += kTileThreads) {
+        const int i = r0 + lr;
+        const float d = inner > 0 ? zf[lr] : s_b[lr];
+        if (mode == 0) d_new[i] = d;
+        else if (mode == 1) x[i] = (double)d;
+        else x[i] += (double)d;
+    }
+}
+
 __device__ __forceinline__ unsigned long long* norm_slot(const NormArgs& N, int member, int word) {
     return N.norms + ((size_t)member * kNormReps + blockIdx.x % kNormReps) * kNormStride + word;
 }
@@ -334,25 +507,27 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
     }
 }
 
-// First refinement step (x = 0): rs = (float)(b / a_ii).
+// First refinement step (x = 0): rs = (float)(b / a_ii) on the rows of the
+// tiles (one workgroup per tile; the rows of a batch's other members are
+// left alone), reading the contiguous diagonal copy.
 template <int K>
 __global__ __launch_bounds__(256) void k_jac_first(TileArgs A, const double* __restrict__ b, float* rs) {
-    constexpr int lgK = kLog2<K>();
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long row = idx >> lgK;
-    const int rr = (int)(idx & (K - 1));
-    if (row >= A.n || rr >= A.k) return;
-    const size_t i = (size_t)row * A.k + rr;
-    rs[i] = (float)(b[i] / A.a64[A.ia[row + 1] - 1]);
+    const int4 tl = A.tiles[blockIdx.x];
+    const int k = A.k;
+    const long long begin = (long long)tl.x * k, end = (long long)tl.y * k;
+    for (long long i = begin + threadIdx.x; i < end; i += blockDim.x) rs[i] = (float)(b[i] / A.diag[i / k]);
 }
 
-// sval = a_ij / a_ii in FP32 for j < i, 0 on the diagonal (one row per thread).
-__global__ void k_jac_scale(int n, const int* __restrict__ ia, const double* __restrict__ a, float* sval) {
+// sval = a_ij / a_ii in FP32 for j < i, 0 on the diagonal, and diag = a_ii
+// (one row per thread).
+__global__ void k_jac_scale(int n, const int* __restrict__ ia, const double* __restrict__ a, float* sval,
+                            double* diag) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int rs = ia[i], re = ia[i + 1];
         const double dg = a[re - 1];
         for (int p = rs; p < re - 1; ++p) sval[p] = (float)(a[p] / dg);
         sval[re - 1] = 0.0f;
+        diag[i] = dg;
     }
 }
 
@@ -540,8 +715,10 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
     }
     if (!pv->jac_sval) {
         SPT_HIP(hipMalloc(&pv->jac_sval, sizeof(float) * (size_t)std::max(1, m->nnz)));
+        SPT_HIP(hipMalloc(&pv->jac_diag, sizeof(double) * (size_t)std::max(1, m->nr)));
         const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(4096, ((long long)m->nr + 255) / 256));
-        hipLaunchKernelGGL(k_jac_scale, dim3(g), dim3(256), 0, nullptr, m->nr, m->ia, m->a, pv->jac_sval);
+        hipLaunchKernelGGL(k_jac_scale, dim3(g), dim3(256), 0, nullptr, m->nr, m->ia, m->a, pv->jac_sval,
+                           pv->jac_diag);
         SPT_HIP(hipGetLastError());
         SPT_HIP(hipDeviceSynchronize());
     }
@@ -617,7 +794,14 @@ int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vec
 hipEvent_t* kernel_timer_pair(hipStream_t s);
 template <int K, int TW>
 static void launch_sweep(const TileArgs& AS, int ntiles, const float* rs, const float* cur, float* nxt, double* x,
-                         int mode, int epl, hipStream_t s) {
+                         int mode, int epl, hipStream_t s, int inner = -1) {
+    if constexpr (K == 1) {
+        if (inner >= 0) {
+            hipLaunchKernelGGL((k_jac_bsweep<TW>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x, mode,
+                               epl, inner);
+            return;
+        }
+    }
     if (AS.sja)
         hipLaunchKernelGGL((k_jac_sweep<K, TW, true>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x,
                            mode, epl);
@@ -648,11 +832,13 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     const TileTable& t0 = pv->jac_tt[0];
     const int wide = (c.lanes == 2 && pv->jac_tt[1].count > 0) ? 1 : 0;  // sweep tiles: 1024 or 2048
     const TileTable& ts = pv->jac_tt[wide];
-    TileArgs A{m->nr, k, m->ia, m->ja, pv->jac_sval, m->a, reinterpret_cast<const int4*>(t0.tiles), t0.member,
-               nullptr, nullptr, nullptr};
+    TileArgs A{m->nr, k, m->ia, m->ja, pv->jac_sval, m->a, pv->jac_diag, reinterpret_cast<const int4*>(t0.tiles),
+               t0.member, nullptr, nullptr, nullptr};
     TileArgs AS = A;
     AS.tiles = reinterpret_cast<const int4*>(ts.tiles);
     AS.tile_member = ts.member;
+    // block sweeps: in-tile iterations per sweep (-1: point sweeps)
+    const int inner = (c.variant == 2 && K == 1) ? (c.inner > 0 ? c.inner : 4) : -1;
     if (c.variant == 1 && ts.sja) {  // column-sorted tiles
         AS.sja = ts.sja;
         AS.ssv = ts.ssv;
@@ -663,7 +849,6 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     const NormArgs with{pv->jac_norms, std::max(1, pv->jac_members)};
     if (t0.count == 0) return SPTRSV_OK;
     const unsigned tiles = (unsigned)t0.count;
-    const unsigned first_grid = ceil_div((long long)m->nr * K, 256);
     const int sweeps_first = std::max(2, c.sweeps);
     const int sweeps_later = c.sweeps_next > 0 ? std::max(2, c.sweeps_next) : sweeps_first;
     const int outer = std::max(1, c.outer);
@@ -671,7 +856,7 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     float* rs = pv->jac_r;  // D^-1 r of the current step; also the first iterate
     for (int o = 0; o < outer; ++o) {
         if (o == 0)
-            hipLaunchKernelGGL((k_jac_first<K>), dim3(first_grid), dim3(256), 0, s, A, b, rs);
+            hipLaunchKernelGGL((k_jac_first<K>), dim3(tiles), dim3(256), 0, s, A, b, rs);
         else
             hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b,
                                (const double*)x, rs, none, epl);
@@ -684,9 +869,9 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
             hipEvent_t* ev = kernel_timer_pair(s);
             if (ev) SPT_HIP(hipEventRecord(ev[0], s));
             if (wide)
-                launch_sweep<K, 2048>(AS, ts.count, rs, cur, nxt, x, mode, epl, s);
+                launch_sweep<K, 2048>(AS, ts.count, rs, cur, nxt, x, mode, epl, s, inner);
             else
-                launch_sweep<K, 1024>(AS, ts.count, rs, cur, nxt, x, mode, epl, s);
+                launch_sweep<K, 1024>(AS, ts.count, rs, cur, nxt, x, mode, epl, s, inner);
             if (ev) SPT_HIP(hipEventRecord(ev[1], s));
             cur = nxt;
             nxt = (nxt == pv->jac_a) ? pv->jac_b : pv->jac_a;
@@ -732,7 +917,8 @@ int jacobi_values_changed(sp_mat_t* m) {
     pv->margin_known = false;
     if (pv->jac_sval && m->nr > 0) {
         const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(4096, ((long long)m->nr + 255) / 256));
-        hipLaunchKernelGGL(k_jac_scale, dim3(g), dim3(256), 0, nullptr, m->nr, m->ia, m->a, pv->jac_sval);
+        hipLaunchKernelGGL(k_jac_scale, dim3(g), dim3(256), 0, nullptr, m->nr, m->ia, m->a, pv->jac_sval,
+                           pv->jac_diag);
         SPT_HIP(hipGetLastError());
         for (TileTable& t : pv->jac_tt)
             if (t.sja && t.count > 0) {
@@ -766,30 +952,48 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
     float bt = 1e30f;
     sptrsv_config_t bc{};
     int st = SPTRSV_OK;
+    // whether schedule c certifies tol/8 on the hashed right-hand side
+    auto certifies = [&](const sptrsv_config_t& c) -> bool {
+        int zero = 0, flag = 0;
+        if (hipMemcpy(pv->counters + 3, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            st = fail(SPTRSV_E_HIP, "clear flag");
+            return false;
+        }
+        pv->tol = tol / 8;
+        st = jacobi_solve(m, c, k, b, x, nullptr);
+        pv->tol = tol;
+        if (st == SPTRSV_OK && hipMemcpy(&flag, pv->counters + 3, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+            st = fail(SPTRSV_E_HIP, "read flag");
+        return st == SPTRSV_OK && flag == 0;
+    };
+    // point sweeps (variant 0) and, for one right-hand side, block sweeps
+    // (variant 2: exact solves of the tiles' diagonal blocks)
+    const std::vector<int> point_sweeps{4, 6, 8, 10, 12, 14, 16, 20, 24};
+    const std::vector<int> block_sweeps{2, 3, 4, 5, 6, 8, 10, 12, 16};
+    for (int var : {0, 2}) {
+    if (var == 2 && k != 1) break;
     int prev_outer = 16;
-    for (int sweeps : {4, 6, 8, 10, 12, 14, 16, 20, 24}) {
-        sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, 1, 256, 1, 0, 0, sweeps, 1, 0};
+    for (int sweeps : (var == 2 ? block_sweeps : point_sweeps)) {
+        sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, 1, 256, 1, var, 0, sweeps, 1, 0, 0, var == 2 ? 4 : 0};
         // the smallest number of refinement steps that certifies tol/8
         bool found = false;
-        pv->tol = tol / 8;
         for (int outer = 1; outer <= std::min(16, prev_outer) && st == SPTRSV_OK; ++outer) {
             c.outer = outer;
-            int zero = 0;
-            SPT_HIP(hipMemcpy(pv->counters + 3, &zero, sizeof(int), hipMemcpyHostToDevice));
-            st = jacobi_solve(m, c, k, b, x, nullptr);
-            int flag = 0;
-            SPT_HIP(hipMemcpy(&flag, pv->counters + 3, sizeof(int), hipMemcpyDeviceToHost));
-            if (st == SPTRSV_OK && flag == 0) {
+            if (certifies(c)) {
                 found = true;
                 break;
             }
         }
-        pv->tol = tol;
         if (st != SPTRSV_OK) break;
         if (!found) continue;
         prev_outer = c.outer;  // more sweeps per step never need more steps
         for (int width : {1, 2}) {  // sweep tiles of 1024 or 2048 products
             c.lanes = width;
+            // block sweeps iterate differently with another tile width
+            if (var == 2 && width == 2 && !certifies(c)) {
+                if (st != SPTRSV_OK) break;
+                continue;
+            }
             st = jacobi_solve(m, c, k, b, x, nullptr);
             SPT_HIP(hipEventRecord(e0, nullptr));
             for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
@@ -805,15 +1009,19 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
         }
         if (st != SPTRSV_OK) break;
     }
-    // column-sorted tiles, then phase-2 entries per lane (default 8), each
-    // on the best schedule so far
+    if (st != SPTRSV_OK) break;
+    }
+    // column-sorted tiles (point sweeps), then phase-2 entries per lane
+    // (default 8), each on the best schedule so far
     for (int v : {-1, 2, 4, 16}) {
         if (st != SPTRSV_OK || bt >= 1e30f) break;
         sptrsv_config_t c = bc;
-        if (v < 0)
+        if (v < 0) {
+            if (bc.variant != 0) continue;
             c.variant = 1;  // column-sorted tiles
-        else
+        } else {
             c.window = v;
+        }
         st = jacobi_solve(m, c, k, b, x, nullptr);
         SPT_HIP(hipEventRecord(e0, nullptr));
         for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
@@ -830,17 +1038,31 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
     // fewer sweeps in the steps after the first: they solve for a correction
     // that needs only the remaining relative accuracy; stop at the first
     // count that no longer certifies tol/8
+    // block sweeps: other in-tile iteration counts on the best schedule
+    for (int inn : {2, 8}) {
+        if (bc.variant != 2 || bt >= 1e30f || st != SPTRSV_OK) break;
+        sptrsv_config_t c = bc;
+        c.inner = inn;
+        if (!certifies(c)) {
+            if (st != SPTRSV_OK) break;
+            continue;
+        }
+        SPT_HIP(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
+        SPT_HIP(hipEventRecord(e1, nullptr));
+        SPT_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= std::max(reps, 1);
+        if (st == SPTRSV_OK && ms < bt) {
+            bt = ms;
+            bc = c;
+        }
+    }
     for (int s2 = bc.sweeps - 1; s2 >= 2 && bc.outer > 1 && bt < 1e30f && st == SPTRSV_OK; --s2) {
         sptrsv_config_t c = bc;
         c.sweeps_next = s2;
-        int zero = 0;
-        SPT_HIP(hipMemcpy(pv->counters + 3, &zero, sizeof(int), hipMemcpyHostToDevice));
-        pv->tol = tol / 8;
-        st = jacobi_solve(m, c, k, b, x, nullptr);
-        pv->tol = tol;
-        int flag = 0;
-        SPT_HIP(hipMemcpy(&flag, pv->counters + 3, sizeof(int), hipMemcpyDeviceToHost));
-        if (st != SPTRSV_OK || flag != 0) break;
+        if (!certifies(c)) break;
         SPT_HIP(hipEventRecord(e0, nullptr));
         for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
         SPT_HIP(hipEventRecord(e1, nullptr));

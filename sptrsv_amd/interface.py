@@ -248,17 +248,17 @@ def csr_L_solve_mixed(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, ref
 
 
 def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0, sweeps=0, outer=0,
-         window=0, sweeps_next=0) -> Config:
-    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window, sweeps_next)
+         window=0, sweeps_next=0, inner=0) -> Config:
+    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window, sweeps_next, inner)
 
 
 def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
           kernel: int = KERNEL_AUTO, lanes: int = 0, wg: int = 0, fused: int = 1, variant: int = 0,
           cu_waves: int = 0, sweeps: int = 0, outer: int = 0, window: int = 0, sweeps_next: int = 0,
-          stream=None, cfg: Config | None = None) -> None:
+          inner: int = 0, stream=None, cfg: Config | None = None) -> None:
     """cfg, when given, is used as is (e.g. the result of autotune)."""
     cfg = cfg if cfg is not None else _cfg(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window,
-                                           sweeps_next)
+                                           sweeps_next, inner)
     check(L.lib().sptrsv_solve(byref(mat), byref(info) if info is not None else None, precision, byref(cfg),
                                b.ptr, x.ptr, mat.nr, k, refine_steps, stream), "sptrsv_solve")
 
@@ -484,6 +484,15 @@ class Batch:
     def check(self, stream=None) -> int:
         """Status of the last solves (0, E_WATCHDOG or E_NOT_CONVERGED) without raising."""
         return L.lib().sptrsv_batch_check(self.h, stream)
+
+    def schedule(self) -> int:
+        """1 when a MIXED batched solve runs its Jacobi and sync-free halves one after the other."""
+        a = c_int()
+        check(L.lib().sptrsv_batch_schedule(self.h, byref(a)), "sptrsv_batch_schedule")
+        return a.value
+
+    def set_schedule(self, serial: int) -> None:
+        check(L.lib().sptrsv_batch_set_schedule(self.h, serial), "sptrsv_batch_set_schedule")
 
     def close(self) -> None:
         if self.h is not None:
