@@ -219,6 +219,8 @@ def run_batch(args, I, O) -> int:
                                 "outer": jc.outer,
                                 "block_inner_iterations": (jc.inner or 4) if jc.variant == 2 else 0} if jc else None,
             "jacobi_nnz_fraction": round(bytes_jac_rows / nnz, 4),
+            "schedule": dict(zip(("serial_halves", "sync_free_members_as_one_analysed_matrix"),
+                                 (bool(v) for v in bt.schedule()))),
             "rel_err_mixed_max_member": max(errs),
             "f64_batched_one_launch_ms": round(ms64b, 3),
             "f64_sequential_ms": round(ms_seq, 2),

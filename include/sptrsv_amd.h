@@ -335,11 +335,15 @@ int sptrsv_batch_solve_graph(sptrsv_batch_t* bt, int precision, const double* b,
                              double* x, hipStream_t stream);
 int sptrsv_batch_row_offsets(const sptrsv_batch_t* bt, int64_t* row_off /* nmat+1 */);
 int sptrsv_batch_choice(const sptrsv_batch_t* bt, int m, sptrsv_config_t* cfg);
-/* How a MIXED batched solve runs its two halves (set by sptrsv_batch_tune,
- * or by the caller): serial 1 runs the Jacobi members, then the sync-free
- * ones, on the caller's stream; 0 runs them concurrently on two streams. */
-int sptrsv_batch_schedule(const sptrsv_batch_t* bt, int* serial);
-int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial);
+/* How a MIXED batched solve runs (set by sptrsv_batch_tune, or by the
+ * caller after tuning): serial 1 runs the Jacobi members, then the sync-free
+ * ones, on the caller's stream; 0 runs them concurrently on two streams.
+ * separate 1 solves the sync-free members as one matrix of their own, with
+ * its own level analysis and tuned kernel (rows gathered from and scattered
+ * back to the batch order); 0 launches them as ticketed tasks.  A persistent
+ * tuned kernel for the separate matrix needs serial = 1. */
+int sptrsv_batch_schedule(const sptrsv_batch_t* bt, int* serial, int* separate);
+int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial, int separate);
 /* sptrsv_check for the batch: watchdog, or a Jacobi member that missed its
  * certified bound (each member is certified separately). */
 int sptrsv_batch_check(sptrsv_batch_t* bt, hipStream_t stream);

@@ -19,6 +19,8 @@
 // the other members, and both halves are captured into the same graph.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -33,6 +35,9 @@ int upload_concat(const int* n, const int* const* ia, const int* const* ja, cons
 int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, double* x, hipStream_t s);
 int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best_ms);
 int reserve_jacobi(sp_mat_t* m, int k);
+int solve_checked(sp_mat_t* m, dfr_analysis_info_t* info, int precision, const sptrsv_config_t* cfg,
+                  const void* b, void* x, int n, int k, int refine_steps, hipStream_t s);
+int convert_values_to_f32(sp_mat_t* m);
 void jacobi_host_tiles(const int* ia, int r_begin, int r_end, long long row_base, long long entry_base,
                        int window, std::vector<int>& out);
 int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vector<int>* tile_member,
@@ -67,6 +72,32 @@ __global__ __launch_bounds__(WG) void k_batch(SolveArgs A, const Task* __restric
     sf_rows<P, 1, 4>(A, tk.lanes, lane, row, s_pub, tk.start, hi);
 }
 
+__global__ void k_rows_iota(int* rows, long long count, int first) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x)
+        rows[i] = first + (int)i;
+}
+
+__global__ void k_gather_rows(const double* __restrict__ src, const int* __restrict__ rows, int n,
+                              double* __restrict__ dst) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[rows[i]];
+}
+
+__global__ void k_scatter_rows(const double* __restrict__ src, const int* __restrict__ rows, int n,
+                               double* __restrict__ dst) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[rows[i]] = src[i];
+}
+
+__global__ void k_shift(int* p, long long count, int off) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] += off;
+}
+
 template <class P>
 __global__ void k_batch_prepare(P* pub, size_t count, P unsolved, int* ticket) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,6 +126,18 @@ struct sptrsv_batch {
     int ntasks_sf = 0;
     hipStream_t aux = nullptr;           // Jacobi half of a MIXED solve
     bool serial = false;                 // MIXED: Jacobi half, then the sync-free half, on one stream
+    std::vector<long long> ent_off;      // nmat+1: first entry of each matrix
+    std::vector<int> max_row;            // per matrix: longest row
+    // MIXED: the sync-free matrices as one matrix of their own, with its own
+    // level analysis and tuned configuration; its rows are gathered from and
+    // scattered back to the batch order (null: the ticketed task launch)
+    sp_mat_t sub{};
+    dfr_analysis_info_t* sub_info = nullptr;
+    sptrsv_config_t sub_cfg{};
+    int* sub_rows = nullptr;             // batch row of each row of sub
+    double* sub_b = nullptr;
+    double* sub_x = nullptr;
+    bool use_sub = false;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     Task* d_sub = nullptr;               // tuning: tasks of one matrix
     int sub_cap = 0;
@@ -225,25 +268,109 @@ int batch_solve_tasks(sptrsv_batch_t* bt, int precision, const Task* tasks, int 
     return fail(SPTRSV_E_INVALID_ARG, "batched solves take FP64 or MIXED precision");
 }
 
+// The sync-free matrices of a MIXED solve: the separate matrix (gather b,
+// solve with its tuned configuration, scatter x) or the ticketed task launch.
+int sync_free_half(sptrsv_batch_t* bt, int precision, const double* b, double* x, hipStream_t s) {
+    if (bt->use_sub && bt->sub.nr > 0) {
+        const int n = bt->sub.nr;
+        const unsigned g = (unsigned)std::min<long long>(8192, ((long long)n + 255) / 256);
+        hipLaunchKernelGGL(k_gather_rows, dim3(g), dim3(256), 0, s, b, bt->sub_rows, n, bt->sub_b);
+        SPT_HIP(hipGetLastError());
+        SPT_TRY(solve_checked(&bt->sub, bt->sub_info, precision, &bt->sub_cfg, bt->sub_b, bt->sub_x, n, 1, 1, s));
+        hipLaunchKernelGGL(k_scatter_rows, dim3(g), dim3(256), 0, s, bt->sub_x, bt->sub_rows, n, x);
+        SPT_HIP(hipGetLastError());
+        return SPTRSV_OK;
+    }
+    if (bt->ntasks_sf > 0) return batch_solve_tasks(bt, precision, bt->d_tasks_sf, bt->ntasks_sf, b, x, s);
+    return SPTRSV_OK;
+}
+
 // The whole batched solve: FP64 runs every matrix sync-free; MIXED runs the
 // Jacobi matrices on the auxiliary stream while the sync-free ones run on s.
 int batch_solve_all(sptrsv_batch_t* bt, int precision, const double* b, double* x, hipStream_t s) {
-    if (precision != SPTRSV_PREC_MIXED || bt->n_jacobi == 0)
-        return batch_solve_tasks(bt, precision, bt->d_tasks, precision == SPTRSV_PREC_MIXED ? bt->ntasks_sf
-                                                                                           : bt->ntasks,
-                                 b, x, s);
+    if (precision != SPTRSV_PREC_MIXED) return batch_solve_tasks(bt, precision, bt->d_tasks, bt->ntasks, b, x, s);
+    if (bt->n_jacobi == 0) return sync_free_half(bt, precision, b, x, s);
     if (bt->serial) {  // the two halves one after the other, each with the whole device
         SPT_TRY(jacobi_solve(&bt->mat, bt->jac_cfg, 1, b, x, s));
-        if (bt->ntasks_sf > 0) SPT_TRY(batch_solve_tasks(bt, precision, bt->d_tasks_sf, bt->ntasks_sf, b, x, s));
-        return SPTRSV_OK;
+        return sync_free_half(bt, precision, b, x, s);
     }
     SPT_HIP(hipEventRecord(bt->fork_ev, s));
     SPT_HIP(hipStreamWaitEvent(bt->aux, bt->fork_ev, 0));
     SPT_TRY(jacobi_solve(&bt->mat, bt->jac_cfg, 1, b, x, bt->aux));
     SPT_HIP(hipEventRecord(bt->join_ev, bt->aux));
-    if (bt->ntasks_sf > 0) SPT_TRY(batch_solve_tasks(bt, precision, bt->d_tasks_sf, bt->ntasks_sf, b, x, s));
+    SPT_TRY(sync_free_half(bt, precision, b, x, s));
     SPT_HIP(hipStreamWaitEvent(s, bt->join_ev, 0));
     return SPTRSV_OK;
+}
+
+void free_sub(sptrsv_batch_t* bt) {
+    if (bt->sub.priv) free_device_analysis(&bt->sub, bt->sub_info);
+    if (bt->sub_info) free(bt->sub_info);
+    bt->sub_info = nullptr;
+    for (void* p : {(void*)bt->sub_rows, (void*)bt->sub_b, (void*)bt->sub_x})
+        if (p) (void)hipFree(p);
+    bt->sub_rows = nullptr;
+    bt->sub_b = bt->sub_x = nullptr;
+    memset(&bt->sub, 0, sizeof(bt->sub));
+    bt->use_sub = false;
+}
+
+// The sync-free matrices (choice other than JACOBI) copied on the device
+// into one matrix of their own, in batch order, with its level analysis.
+int build_sub(sptrsv_batch_t* bt) {
+    free_sub(bt);
+    const int nmat = (int)bt->choice.size();
+    long long rows = 0, nz = 0;
+    int mr = 0;
+    for (int m = 0; m < nmat; ++m) {
+        if (bt->choice[m] == SPTRSV_KERNEL_JACOBI) continue;
+        rows += bt->row_off[m + 1] - bt->row_off[m];
+        nz += bt->ent_off[m + 1] - bt->ent_off[m];
+        mr = std::max(mr, bt->max_row[m]);
+    }
+    if (rows == 0) return SPTRSV_OK;
+    sp_mat_t& S = bt->sub;
+    SPT_TRY(ensure_priv(&S));
+    S.nr = S.nc = (int)rows;
+    S.nnz = (int)nz;
+    priv_of(&S)->max_row = mr;
+    int st = SPTRSV_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && st == SPTRSV_OK) st = hip_fail(e, what, __FILE__, __LINE__);
+    };
+    chk(hipMalloc(&S.ia, sizeof(int) * (size_t)(rows + 1)), "hipMalloc ia");
+    chk(hipMalloc(&S.ja, sizeof(int) * (size_t)nz), "hipMalloc ja");
+    chk(hipMalloc(&S.a, sizeof(double) * (size_t)nz), "hipMalloc a");
+    chk(hipMalloc(&S.a32, sizeof(float) * (size_t)nz), "hipMalloc a32");
+    chk(hipMalloc(&bt->sub_rows, sizeof(int) * (size_t)rows), "hipMalloc row map");
+    chk(hipMalloc(&bt->sub_b, sizeof(double) * (size_t)rows), "hipMalloc b");
+    chk(hipMalloc(&bt->sub_x, sizeof(double) * (size_t)rows), "hipMalloc x");
+    long long r = 0, e = 0;
+    for (int m = 0; m < nmat && st == SPTRSV_OK; ++m) {
+        if (bt->choice[m] == SPTRSV_KERNEL_JACOBI) continue;
+        const long long r0 = bt->row_off[m], nr = bt->row_off[m + 1] - r0;
+        const long long e0 = bt->ent_off[m], ne = bt->ent_off[m + 1] - e0;
+        if (nr == 0) continue;
+        const unsigned gr = (unsigned)std::min<long long>(4096, (nr + 255) / 256);
+        const unsigned ge = (unsigned)std::min<long long>(4096, (ne + 255) / 256);
+        chk(hipMemcpy(S.ia + r, bt->mat.ia + r0, sizeof(int) * (size_t)nr, hipMemcpyDeviceToDevice), "copy ia");
+        chk(hipMemcpy(S.ja + e, bt->mat.ja + e0, sizeof(int) * (size_t)ne, hipMemcpyDeviceToDevice), "copy ja");
+        chk(hipMemcpy(S.a + e, bt->mat.a + e0, sizeof(double) * (size_t)ne, hipMemcpyDeviceToDevice), "copy a");
+        if (st != SPTRSV_OK) break;
+        hipLaunchKernelGGL(k_shift, dim3(gr), dim3(256), 0, nullptr, S.ia + r, nr, (int)(e - e0));
+        hipLaunchKernelGGL(k_shift, dim3(ge), dim3(256), 0, nullptr, S.ja + e, ne, (int)(r - r0));
+        hipLaunchKernelGGL(k_rows_iota, dim3(gr), dim3(256), 0, nullptr, bt->sub_rows + r, nr, (int)r0);
+        chk(hipGetLastError(), "row block kernels");
+        r += nr;
+        e += ne;
+    }
+    const int last = (int)nz;
+    if (st == SPTRSV_OK) chk(hipMemcpy(S.ia + rows, &last, sizeof(int), hipMemcpyHostToDevice), "ia end");
+    if (st == SPTRSV_OK) st = convert_values_to_f32(&S);
+    if (st == SPTRSV_OK) chk(hipDeviceSynchronize(), "sub build");
+    if (st == SPTRSV_OK) st = multirow_analysis_base_GPU(&bt->sub_info, &S, ALL);
+    if (st != SPTRSV_OK) free_sub(bt);
+    return st;
 }
 
 // Mean time of `reps` whole batched MIXED solves (after one warm-up).
@@ -286,10 +413,14 @@ int sptrsv_batch_create(int nmat, const int* n, const int* const* ia, const int*
     bt->mtiles[1].resize(nmat);
     bt->choice.assign(nmat, SPTRSV_KERNEL_SIMPLE);
     bt->t_sync.assign(nmat, 0.0f);
+    bt->ent_off.assign(nmat + 1, 0);
+    bt->max_row.assign(nmat, 0);
     long long entry_base = 0;
     for (int m = 0; m < nmat; ++m) {
         long long rows = bt->row_off[m + 1] - bt->row_off[m];
         long long nz = rows > 0 ? (long long)ia[m][n[m]] - ia[m][0] : 0;
+        bt->ent_off[m] = entry_base;
+        for (int i = 0; i < n[m]; ++i) bt->max_row[m] = std::max(bt->max_row[m], ia[m][i + 1] - ia[m][i]);
         bt->avg[m] = rows > 0 ? (int)(nz / rows) : 0;
         bt->lanes[m] = lanes_rule(bt->avg[m]);
         double dm = rows > 0 ? 1e300 : 1.0;
@@ -305,6 +436,7 @@ int sptrsv_batch_create(int nmat, const int* n, const int* const* ia, const int*
                 jacobi_host_tiles(ia[m], 0, n[m], bt->row_off[m], entry_base, 1024 << w, bt->mtiles[w][m]);
         entry_base += nz;
     }
+    bt->ent_off[nmat] = entry_base;
     st = build_tasks(bt);
     if (st == SPTRSV_OK) st = sptrsv_reserve(&bt->mat, 1);
     if (st == SPTRSV_OK) st = install_jacobi(bt);
@@ -340,15 +472,23 @@ int sptrsv_batch_choice(const sptrsv_batch_t* bt, int m, sptrsv_config_t* cfg) {
     return SPTRSV_OK;
 }
 
-int sptrsv_batch_schedule(const sptrsv_batch_t* bt, int* serial) {
-    if (!bt || !serial) return fail(SPTRSV_E_INVALID_ARG, "null argument");
+int sptrsv_batch_schedule(const sptrsv_batch_t* bt, int* serial, int* separate) {
+    if (!bt || !serial || !separate) return fail(SPTRSV_E_INVALID_ARG, "null argument");
     *serial = bt->serial ? 1 : 0;
+    *separate = bt->use_sub ? 1 : 0;
     return SPTRSV_OK;
 }
 
-int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial) {
-    if (!bt || serial < 0 || serial > 1) return fail(SPTRSV_E_INVALID_ARG, "bad batch schedule");
+int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial, int separate) {
+    if (!bt || serial < 0 || serial > 1 || separate < 0 || separate > 1)
+        return fail(SPTRSV_E_INVALID_ARG, "bad batch schedule");
+    if (separate && !bt->sub.priv)
+        return fail(SPTRSV_E_INVALID_ARG, "no separate sync-free matrix: tune the batch (MIXED) first");
+    if (separate && !serial && bt->n_jacobi > 0 && bt->sub_cfg.variant == 2 &&
+        bt->sub_cfg.kernel != SPTRSV_KERNEL_JACOBI)
+        return fail(SPTRSV_E_INVALID_ARG, "a persistent configuration runs only with serial halves");
     bt->serial = serial != 0;
+    bt->use_sub = separate != 0;
     for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
     bt->graphs.clear();
     return SPTRSV_OK;
@@ -423,10 +563,10 @@ int apply_choice(sptrsv_batch_t* bt, const std::vector<int>& jac, float* jac_ms)
 // MIXED per-matrix choice between the sync-free launch and Jacobi.  The
 // Jacobi time J of all diagonally dominant matrices is measured first; a
 // dominant matrix whose sync-free time alone exceeds J / 2 is "slow".  Then
-// three whole concurrent batches are timed -- every matrix sync-free,
-// Jacobi for every dominant matrix, Jacobi for the slow ones only -- and the
-// fastest is kept.  Lanes per row are tuned only for matrices that can stay
-// sync-free.
+// whole concurrent batches are timed -- every matrix sync-free, Jacobi for
+// every dominant matrix, for the slow ones, and for those whose sync-free
+// time alone exceeds 1 ms or 0.25 ms -- and the fastest is kept.  Lanes per
+// row are tuned only for matrices that can stay sync-free.
 int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
     const int nmat = (int)bt->choice.size();
     std::vector<int> dominant;
@@ -466,29 +606,70 @@ int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
     SPT_TRY(st);
     std::vector<int> best_set;
     if (jall > 0) {
-        for (const std::vector<int>* set : {&dominant, &slow}) {
-            if (set->empty() || (set == &slow && slow.size() == dominant.size())) continue;
-            SPT_TRY(apply_choice(bt, *set, nullptr));
+        // candidate Jacobi sets: every dominant matrix, and the dominant
+        // matrices whose sync-free time alone exceeds a threshold (a long
+        // dependency chain bounds the sync-free half however few rows it has)
+        std::vector<std::vector<int>> sets{dominant, slow};
+        for (float tau : {1.0f, 0.25f}) {
+            std::vector<int> set;
+            for (int m : dominant)
+                if (bt->t_sync[m] > tau) set.push_back(m);
+            sets.push_back(set);
+        }
+        std::sort(sets.begin(), sets.end());
+        sets.erase(std::unique(sets.begin(), sets.end()), sets.end());
+        for (const std::vector<int>& set : sets) {
+            if (set.empty()) continue;
+            SPT_TRY(apply_choice(bt, set, nullptr));
             const float t = time_batch(bt, ts.b, ts.x, 3, &st);
             SPT_TRY(st);
             if (t < best) {
                 best = t;
-                best_set = *set;
+                best_set = set;
             }
         }
     }
     SPT_TRY(apply_choice(bt, best_set, nullptr));
-    // the two halves concurrently on two streams, or one after the other
-    // (the concurrent Jacobi sweeps can starve the latency-bound sync-free
-    // waves)
+    // The sync-free matrices either through the ticketed task launch or as a
+    // matrix of their own (level analysis, tuned kernel); the two halves
+    // concurrently on two streams or one after the other (the concurrent
+    // sweeps can starve latency-bound waves).  A persistent configuration of
+    // the separate matrix needs every one of its workgroups resident, so it
+    // only runs one after the other.
     bt->serial = false;
-    if (bt->n_jacobi > 0 && bt->ntasks_sf > 0) {
-        const float tc = time_batch(bt, ts.b, ts.x, 5, &st);
-        SPT_TRY(st);
-        bt->serial = true;
-        const float tsr = time_batch(bt, ts.b, ts.x, 5, &st);
-        SPT_TRY(st);
-        bt->serial = tsr < tc;
+    bt->use_sub = false;
+    if (bt->ntasks_sf > 0) {
+        SPT_TRY(build_sub(bt));
+        bool sub_ok = bt->sub.nr > 0;
+        if (sub_ok) {
+            float ms = 0;
+            const int ast = sptrsv_autotune(&bt->sub, bt->sub_info, SPTRSV_PREC_MIXED, 1, 3, &bt->sub_cfg, &ms);
+            if (ast != SPTRSV_OK) {
+                free_sub(bt);
+                sub_ok = false;
+            }
+        }
+        const bool sub_persistent = sub_ok && bt->sub_cfg.variant == 2 && bt->sub_cfg.kernel != SPTRSV_KERNEL_JACOBI;
+        float best_t = 1e30f;
+        bool best_sub = false, best_serial = false;
+        for (int use : {0, 1}) {
+            if (use && !sub_ok) continue;
+            for (int ser : {0, 1}) {
+                if (bt->n_jacobi == 0 && ser) continue;  // nothing to overlap
+                if (use && sub_persistent && !ser && bt->n_jacobi > 0) continue;
+                bt->use_sub = use != 0;
+                bt->serial = ser != 0;
+                const float t = time_batch(bt, ts.b, ts.x, 5, &st);
+                SPT_TRY(st);
+                if (t < best_t) {
+                    best_t = t;
+                    best_sub = bt->use_sub;
+                    best_serial = bt->serial;
+                }
+            }
+        }
+        bt->use_sub = best_sub;
+        bt->serial = best_serial;
     }
     for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
     bt->graphs.clear();
@@ -562,7 +743,9 @@ int sptrsv_batch_solve_graph(sptrsv_batch_t* bt, int precision, const double* b,
 
 int sptrsv_batch_check(sptrsv_batch_t* bt, hipStream_t stream) {
     if (!bt) return fail(SPTRSV_E_INVALID_ARG, "null batch");
-    return sptrsv_check(&bt->mat, stream);
+    const int st = sptrsv_check(&bt->mat, stream);
+    if (st != SPTRSV_OK || !bt->use_sub || !bt->sub.priv) return st;
+    return sptrsv_check(&bt->sub, stream);
 }
 
 void sptrsv_batch_destroy(sptrsv_batch_t* bt) {
@@ -575,6 +758,7 @@ void sptrsv_batch_destroy(sptrsv_batch_t* bt) {
     if (bt->aux) (void)hipStreamDestroy(bt->aux);
     if (bt->fork_ev) (void)hipEventDestroy(bt->fork_ev);
     if (bt->join_ev) (void)hipEventDestroy(bt->join_ev);
+    free_sub(bt);
     free_device_analysis(&bt->mat, nullptr);
     delete bt;
 }

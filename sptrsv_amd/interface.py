@@ -485,14 +485,15 @@ class Batch:
         """Status of the last solves (0, E_WATCHDOG or E_NOT_CONVERGED) without raising."""
         return L.lib().sptrsv_batch_check(self.h, stream)
 
-    def schedule(self) -> int:
-        """1 when a MIXED batched solve runs its Jacobi and sync-free halves one after the other."""
-        a = c_int()
-        check(L.lib().sptrsv_batch_schedule(self.h, byref(a)), "sptrsv_batch_schedule")
-        return a.value
+    def schedule(self) -> tuple[int, int]:
+        """(serial, separate) of a MIXED batched solve: halves one after the
+        other; sync-free members solved as one analysed matrix of their own."""
+        a, b = c_int(), c_int()
+        check(L.lib().sptrsv_batch_schedule(self.h, byref(a), byref(b)), "sptrsv_batch_schedule")
+        return a.value, b.value
 
-    def set_schedule(self, serial: int) -> None:
-        check(L.lib().sptrsv_batch_set_schedule(self.h, serial), "sptrsv_batch_set_schedule")
+    def set_schedule(self, serial: int, separate: int) -> None:
+        check(L.lib().sptrsv_batch_set_schedule(self.h, serial, separate), "sptrsv_batch_set_schedule")
 
     def close(self) -> None:
         if self.h is not None:

@@ -163,6 +163,29 @@ def test_batch_per_member_jacobi_choice():
         for m, s in enumerate(systems):
             xm = x[bt.row_off[m]:bt.row_off[m + 1]]
             assert relerr(xm, refs[m]) <= 1e-10, (m, kinds[m], graph)
+    # every schedule the caller may set gives the same answers: halves
+    # concurrent or serial, sync-free members as tasks or as a separate
+    # analysed matrix (a persistent kernel for it is refused when concurrent)
+    print("tuned schedule:", bt.schedule(), flush=True)
+    for serial in (0, 1):
+        for separate in (0, 1):
+            try:
+                bt.set_schedule(serial, separate)
+            except L.SptrsvError as e:
+                # refused: a persistent separate kernel next to concurrent
+                # Jacobi sweeps, or no sync-free member to separate
+                all_jacobi = all(k == I.KERNEL_JACOBI for k in kinds)
+                assert e.status == L.E_INVALID_ARG and separate == 1 and (serial == 0 or all_jacobi), kinds
+                continue
+            for graph in (False, True):
+                dx.fill_bytes(0)
+                bt.solve(I.PREC_MIXED, db, dx, graph=graph)
+                I.synchronize()
+                assert bt.check() == 0, L.last_error()
+                x = dx.numpy()
+                for m, s in enumerate(systems):
+                    xm = x[bt.row_off[m]:bt.row_off[m + 1]]
+                    assert relerr(xm, refs[m]) <= 1e-10, (m, serial, separate, graph)
     # FP64 batches stay sync-free for every member
     bt.solve(I.PREC_F64, db, dx, graph=False)
     I.synchronize()
