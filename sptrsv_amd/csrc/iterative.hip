@@ -170,11 +170,17 @@ struct TileStage {
                 }
             }
         }
-        // level 3
+        // level 3 (the diagonal from its contiguous copy when there is one,
+        // else gathered from the values through the row pointers)
         double dg[RU];
         if (DIAG) {
 #pragma unroll
-            for (int u = 0; u < RU; ++u) dg[u] = (u * kTileThreads <= rows) ? A.a64[max(iav[u] - 1, 0)] : 1.0;
+            for (int u = 0; u < RU; ++u) {
+                const int t = min(tid + u * kTileThreads, rows);
+                dg[u] = (u * kTileThreads > rows) ? 1.0
+                        : A.diag             ? A.diag[max(r0 + t - 1, 0)]
+                                             : A.a64[max(iav[u] - 1, 0)];
+            }
         }
         P xv[U];
         if (lds) {
