@@ -381,16 +381,27 @@ def main() -> int:
     try:
         if args.no_extras:
             raise RuntimeError("skipped (--no-extras)")
+        # csrsv takes one contiguous right-hand side: with k > 1 it solves
+        # RHS 0, split out of the row-major n x k block
+        if k == 1:
+            rb_b, rb_x, rb_ref = db, dx, x_ref
+        else:
+            b0 = np.ascontiguousarray(s.b.reshape(s.n, k)[:, 0])
+            rb_b, rb_x = I.DeviceArray.from_numpy(b0), I.DeviceArray(s.n, np.float64)
+            rb_ref = np.ascontiguousarray(x_ref.reshape(s.n, k)[:, 0])
         rb = I.RocsparseBaseline(mat)
-        rb.solve(db, dx)
+        rb.solve(rb_b, rb_x)
         I.synchronize()
         timer = I.Timer()
         timer.start()
         for _ in range(args.steps):
-            rb.solve(db, dx)
+            rb.solve(rb_b, rb_x)
         rs_ms = timer.stop() / args.steps
-        x = dx.numpy()
-        rs_err = float(np.max(np.abs(x - x_ref)) / np.max(np.abs(x_ref)))
+        x = rb_x.numpy()
+        rs_err = float(np.max(np.abs(x - rb_ref)) / np.max(np.abs(rb_ref)))
+        if k > 1:
+            rb_b.free()
+            rb_x.free()
         rocsparse = {"ms": round(rs_ms, 4), "gnnz_s": round(k * s.nnz / (rs_ms * 1e-3) / 1e9 if k == 1 else
                                                            s.nnz / (rs_ms * 1e-3) / 1e9, 4),
                      "analysis_ms": round(rb.analysis_ms, 3), "rel_err": rs_err,

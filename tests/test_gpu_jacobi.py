@@ -215,6 +215,19 @@ def test_jacobi_tile_shapes(cfg, param, k):
     d.close()
 
 
+@pytest.mark.parametrize("cfg,param", [(2, 150000), (3, 60000)])
+def test_sorted_tiles_equal_storage_order_bitwise(cfg, param):
+    """Column-sorted tiles only reorder the gathers: every product lands in
+    its storage slot and rows are summed in storage order, so the solve is
+    bit-identical to the storage-order sweep, for either tile width."""
+    s = I.generate(cfg, param)
+    d = Dev(s.ia, s.ja, s.a, analyse=False)
+    for tile in (1, 2):
+        xs = [d.solve(s.b, lanes=tile, variant=v, window=8, sweeps=12, outer=3, **J) for v in (0, 1)]
+        assert np.array_equal(xs[0], xs[1]), tile
+    d.close()
+
+
 def test_graph_captured_solves_replay_exactly():
     """sptrsv_solve_graph: every replay, with eager solves of other
     configurations in between, equals the eager solve bit for bit."""
