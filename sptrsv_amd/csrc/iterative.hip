@@ -479,6 +479,9 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
     const int rpp = kTileThreads / GL;
     const int npass = (T.rows + rpp - 1) / rpp;
     const int gl = tid % GL, e = gl / K, rr = gl & (K - 1);
+    // max |r| and max |x| of this thread's rows, as the bit patterns of
+    // non-negative doubles (ordered like the values)
+    unsigned long long mr = 0ull, mx = 0ull;
     for (int ps = 0; ps < npass; ++ps) {
         const int lr = ps * rpp + tid / GL;
         const bool live = lr < T.rows && rr < k;
@@ -499,12 +502,22 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
             const double r = s_b[lr * K + rr] - s;
             if (rs_out) rs_out[i] = (float)(r / s_dg[lr]);
             if (N.norms) {
-                atomicMax(&s_nrm[0], (unsigned long long)__double_as_longlong(fabs(r)));
-                atomicMax(&s_nrm[1], (unsigned long long)__double_as_longlong(fabs(x[i])));
+                mr = max(mr, (unsigned long long)__double_as_longlong(fabs(r)));
+                mx = max(mx, (unsigned long long)__double_as_longlong(fabs(x[i])));
             }
         }
     }
     if (N.norms) {
+        // wave maxima first: one LDS atomic per wave instead of one per row
+        // (same-address atomics serialise)
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            mr = max(mr, (unsigned long long)__shfl_xor((long long)mr, o));
+            mx = max(mx, (unsigned long long)__shfl_xor((long long)mx, o));
+        }
+        if ((tid & (kWave - 1)) == 0) {
+            if (mr) atomicMax(&s_nrm[0], mr);
+            if (mx) atomicMax(&s_nrm[1], mx);
+        }
         __syncthreads();
         if (tid < 2 && s_nrm[tid] != 0ull) {
             const int m = A.tile_member ? A.tile_member[xcd_tile(blockIdx.x, gridDim.x)] : 0;
