@@ -61,7 +61,7 @@ struct TileArgs {
     const int* ja;
     const float* sval;        // a_ij / a_ii in FP32 (0 on the diagonal)
     const double* a64;
-    const double* diag;       // a_ii, contiguous
+    const double* diag;       // a_ii, contiguous (read by the residual and the first iterate)
     const int4* tiles;        // per tile {first row, end row, first entry, end entry}
     const int* tile_member;   // member (independent system) of each tile; null: one member
     // column-sorted tiles (sweep only): each tile's entries ordered by column,
@@ -109,9 +109,9 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 }
 
 // Stages one tile in LDS with every independent load issued before the
-// first use: level 1 the tile record; level 2 the row pointers, the per-row
-// right-hand side (V) and the entries; level 3 the gathered iterate (and,
-// with DIAG, the diagonals); then all LDS stores.  Products a * v[ja]
+// first use: level 1 the tile record; level 2 the row pointers, (with DIAG)
+// the diagonals from their contiguous copy, the per-row right-hand side (V)
+// and the entries; level 3 the gathered iterate; then all LDS stores.  Products a * v[ja]
 // (diagonal included) go to prod when the tile fits.
 template <int K, int TW, bool DIAG, class V, class P, bool SORTED = false>
 struct TileStage {
@@ -134,11 +134,27 @@ struct TileStage {
         E = tl.w - tl.z;
         if (rows <= 0) return false;
         lds = E * K <= CAP;
+        // Load order: the per-row loads (row pointers, diagonal, right-hand
+        // side), guarded by the tile's row count, then the entries and the
+        // gathers, unconditional with their indices clamped inside the tile
+        // (a slot past the tile rereads its last entry and is never stored).
+        // With every guarded load issued before the entries, the compiler
+        // counts the loads outstanding after each entry exactly, so a gather
+        // waits only for its own entry; a guard between the entry loads and
+        // the gathers made it wait conservatively and serialised the gathers.
         // level 2
         int iav[RU];
+        double dg[RU];
 #pragma unroll
-        for (int u = 0; u < RU; ++u)
-            iav[u] = (u * kTileThreads <= rows) ? A.ia[r0 + min(tid + u * kTileThreads, rows)] : 1;
+        for (int u = 0; u < RU; ++u) {
+            iav[u] = 1;
+            dg[u] = 1.0;
+            if (u * kTileThreads <= rows) {
+                const int t = min(tid + u * kTileThreads, rows);
+                iav[u] = A.ia[r0 + t];
+                if (DIAG) dg[u] = A.diag[max(r0 + t - 1, 0)];
+            }
+        }
         V rv[SU];
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
@@ -149,48 +165,29 @@ struct TileStage {
                 rv[u] = rhs[(size_t)(r0 + lr) * k + rr];
             }
         }
-        int jv[U], slot[U];
-        P av[U];
+        int jv[U] = {}, slot[U] = {};
+        P av[U] = {};
         if (lds) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                jv[u] = 0;
-                av[u] = P(0);
-                slot[u] = 0;
-                if (u * kTileThreads < E * K) {
-                    const int e = min((tid + u * kTileThreads) / K, E - 1);
-                    if (SORTED) {
-                        jv[u] = A.sja[p0 + e];
-                        av[u] = (P)A.ssv[p0 + e];
-                        slot[u] = A.sslot[p0 + e];
-                    } else {
-                        jv[u] = A.ja[p0 + e];
-                        av[u] = vals[p0 + e];
-                    }
+                const int e = min((tid + u * kTileThreads) / K, E - 1);
+                if (SORTED) {
+                    jv[u] = A.sja[p0 + e];
+                    av[u] = (P)A.ssv[p0 + e];
+                    slot[u] = A.sslot[p0 + e];
+                } else {
+                    jv[u] = A.ja[p0 + e];
+                    av[u] = vals[p0 + e];
                 }
             }
         }
-        // level 3 (the diagonal from its contiguous copy when there is one,
-        // else gathered from the values through the row pointers)
-        double dg[RU];
-        if (DIAG) {
-#pragma unroll
-            for (int u = 0; u < RU; ++u) {
-                const int t = min(tid + u * kTileThreads, rows);
-                dg[u] = (u * kTileThreads > rows) ? 1.0
-                        : A.diag             ? A.diag[max(r0 + t - 1, 0)]
-                                             : A.a64[max(iav[u] - 1, 0)];
-            }
-        }
-        P xv[U];
+        // level 3
+        P xv[U] = {};
         if (lds) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                xv[u] = P(0);
-                if (u * kTileThreads < E * K) {
-                    const int rr = min((tid + u * kTileThreads) & (K - 1), k - 1);
-                    xv[u] = v[(size_t)jv[u] * k + rr];
-                }
+                const int rr = min((tid + u * kTileThreads) & (K - 1), k - 1);
+                xv[u] = v[(size_t)jv[u] * k + rr];
             }
         }
         // LDS
