@@ -323,28 +323,35 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const f
         }
         return;
     }
-    // staging: every load issued before its first use
+    // staging: every load issued before its first use.  The per-row loads,
+    // guarded by the row count, come first; the entry loads and the gathers
+    // are unconditional (indices clamped inside the tile; an in-tile column
+    // is gathered too and not used), so each gather waits only for its own
+    // entry (the reasoning of TileStage).
     int iav[RU];
-#pragma unroll
-    for (int u = 0; u < RU; ++u) iav[u] = (u * kTileThreads <= rows) ? A.ia[r0 + min(tid + u * kTileThreads, rows)] : 0;
     float rv[RU], zv[RU];
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
-        const int lr = min(tid + u * kTileThreads, rows - 1);
-        rv[u] = (u * kTileThreads < rows) ? rs[r0 + lr] : 0.0f;
-        zv[u] = (u * kTileThreads < rows) ? d_old[r0 + lr] : 0.0f;
+        iav[u] = 0;
+        rv[u] = zv[u] = 0.0f;
+        if (u * kTileThreads <= rows) iav[u] = A.ia[r0 + min(tid + u * kTileThreads, rows)];
+        if (u * kTileThreads < rows) {
+            const int lr = min(tid + u * kTileThreads, rows - 1);
+            rv[u] = rs[r0 + lr];
+            zv[u] = d_old[r0 + lr];
+        }
     }
     int jv[U];
     float av[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int e = min(tid + u * kTileThreads, E - 1);
-        jv[u] = (u * kTileThreads < E) ? A.ja[p0 + e] : r0;
-        av[u] = (u * kTileThreads < E) ? A.sval[p0 + e] : 0.0f;
+        jv[u] = A.ja[p0 + e];
+        av[u] = A.sval[p0 + e];
     }
     float xv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) xv[u] = (jv[u] < r0) ? d_old[jv[u]] : 0.0f;
+    for (int u = 0; u < U; ++u) xv[u] = d_old[jv[u]];
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
         const int t = tid + u * kTileThreads;
