@@ -107,6 +107,18 @@ def cfg_dict(c) -> dict:
     return d
 
 
+def sweep_layout(n, tw, variant):
+    """Entry layout of the point sweep kernel the library launches (its third
+    template argument): 0 storage order, 1 column-sorted with a separate
+    2-byte position, 2 column-sorted with the position packed above the
+    column, used when n <= 2^(32 - p), p the bits of a position below the
+    tile's LDS capacity of tw + tw/4 products."""
+    if variant != 1:
+        return 0
+    pbits = (tw + tw // 4 - 1).bit_length()
+    return 2 if n <= (1 << (32 - pbits)) else 1
+
+
 def profiled_traffic(kernel_key: str):
     """HBM-side bytes per launch of `kernel_key` from the rocprofv3 FETCH_SIZE /
     WRITE_SIZE summaries committed under profiles/ (calibrated there against a
@@ -453,7 +465,8 @@ def main() -> int:
         kpow = 1
         while kpow < k:
             kpow *= 2
-        key = f"k_jac_sweep<{kpow},{1024 * max(1, cm.lanes)},{'true' if cm.variant == 1 else 'false'}>@workload{args.workload}"
+        tw = 1024 * max(1, cm.lanes)
+        key = f"k_jac_sweep<{kpow},{tw},{sweep_layout(n, tw, cm.variant)}>@workload{args.workload}"
         kname = f"Jacobi sweep (FP32, {key.split('@')[0]}) inside FP64 refinement"
         traffic = profiled_traffic(key)
     else:

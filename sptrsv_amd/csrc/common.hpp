@@ -48,10 +48,12 @@ struct TileTable {
     int count = 0;
     int K = 0;               // right-hand-side width the tiles were cut for
     // entries of each tile sorted by column (built on request): column,
-    // scaled value, position inside the tile
+    // scaled value, position inside the tile (packed above the column's
+    // cbits bits of sja when cbits > 0; else in sslot)
     int* sja = nullptr;
     float* ssv = nullptr;
     unsigned short* sslot = nullptr;
+    int cbits = 0;
 };
 
 struct MatPriv {

@@ -65,10 +65,13 @@ struct TileArgs {
     const int4* tiles;        // per tile {first row, end row, first entry, end entry}
     const int* tile_member;   // member (independent system) of each tile; null: one member
     // column-sorted tiles (sweep only): each tile's entries ordered by column,
-    // with the entry's position inside its tile; null: storage order
+    // with the entry's position inside its tile; null: storage order.  With
+    // cbits > 0 the position is packed above the column's cbits bits of sja
+    // and sslot is unused.
     const int* sja;
     const float* ssv;
     const unsigned short* sslot;
+    int cbits;
 };
 
 struct NormArgs {
@@ -113,7 +116,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 // the diagonals from their contiguous copy, the per-row right-hand side (V)
 // and the entries; level 3 the gathered iterate; then all LDS stores.  Products a * v[ja]
 // (diagonal included) go to prod when the tile fits.
-template <int K, int TW, bool DIAG, class V, class P, bool SORTED = false>
+// SORTED: 0 storage order, 1 column-sorted with positions in sslot, 2
+// column-sorted with positions packed above the column's cbits bits (the
+// word is kept whole until the gather needs its column).
+template <int K, int TW, bool DIAG, class V, class P, int SORTED = 0>
 struct TileStage {
     static constexpr int W = TW / K;                       // entries per window = max rows per tile
     static constexpr int CAP = tile_cap<TW>();
@@ -174,7 +180,7 @@ struct TileStage {
                 if (SORTED) {
                     jv[u] = A.sja[p0 + e];
                     av[u] = (P)A.ssv[p0 + e];
-                    slot[u] = A.sslot[p0 + e];
+                    if (SORTED == 1) slot[u] = A.sslot[p0 + e];
                 } else {
                     jv[u] = A.ja[p0 + e];
                     av[u] = vals[p0 + e];
@@ -187,7 +193,8 @@ struct TileStage {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int rr = min((tid + u * kTileThreads) & (K - 1), k - 1);
-                xv[u] = v[(size_t)jv[u] * k + rr];
+                const int col = (SORTED == 2) ? (int)((unsigned)jv[u] & ((1u << A.cbits) - 1u)) : jv[u];
+                xv[u] = v[(size_t)col * k + rr];
             }
         }
         // LDS
@@ -206,7 +213,8 @@ struct TileStage {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int t = tid + u * kTileThreads;
-                if (t < E * K) prod[SORTED ? slot[u] * K + (t & (K - 1)) : t] = av[u] * xv[u];
+                const int sl = (SORTED == 2) ? (int)((unsigned)jv[u] >> A.cbits) : slot[u];
+                if (t < E * K) prod[SORTED ? sl * K + (t & (K - 1)) : t] = av[u] * xv[u];
             }
         }
         return true;
@@ -216,7 +224,7 @@ struct TileStage {
 // One Jacobi sweep on the row-scaled system: d_new = rs - N' d_old, with
 // N' = D^-1 N stored as sval and rs = D^-1 r.  mode 0 stores d_new, mode 1
 // sets x = d_new (first refinement step, x was 0), mode 2 adds x += d_new.
-template <int K, int TW, bool SORTED>
+template <int K, int TW, int SORTED>
 __device__ __forceinline__ void sweep_tile(const TileArgs& A, int tile, const float* __restrict__ rs,
                                            const float* __restrict__ d_old, float* __restrict__ d_new, double* x,
                                            int mode, int epl, float* prod, int* s_ia, float* s_r) {
@@ -264,7 +272,7 @@ __device__ __forceinline__ void sweep_tile(const TileArgs& A, int tile, const fl
 // N' = D^-1 N stored as sval and rs = D^-1 r.  mode 0 stores d_new, mode 1
 // sets x = d_new (first refinement step, x was 0), mode 2 adds x += d_new.
 // One workgroup per tile.
-template <int K, int TW, bool SORTED>
+template <int K, int TW, int SORTED>
 __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const float* __restrict__ rs,
                                                             const float* __restrict__ d_old,
                                                             float* __restrict__ d_new, double* x, int mode,
@@ -557,11 +565,15 @@ __global__ void k_jac_scale(int n, const int* __restrict__ ia, const double* __r
 // Column-sorted copy of each tile's entries (setup): a bitonic sort of
 // (column, position) pairs in LDS, one workgroup per tile.  Tiles of more
 // than kSortCap entries (summed from memory by the sweep) keep storage order.
+// With cbits > 0 the position is stored in the bits of sja above the
+// column's cbits (the caller guarantees n <= 2^cbits and that a position the
+// sweep reads, below the tile's LDS capacity, fits the 32 - cbits bits left);
+// sslot is then not written.
 constexpr int kSortCap = 4096;
 
 __global__ __launch_bounds__(256) void k_jac_sort_tiles(const int4* __restrict__ tiles, const int* __restrict__ ja,
                                                         const float* __restrict__ sval, int* sja, float* ssv,
-                                                        unsigned short* sslot) {
+                                                        unsigned short* sslot, int cbits) {
     __shared__ unsigned long long key[kSortCap];
     const int4 tl = tiles[blockIdx.x];
     const int p0 = tl.z, E = tl.w - tl.z;
@@ -571,7 +583,7 @@ __global__ __launch_bounds__(256) void k_jac_sort_tiles(const int4* __restrict__
         for (int i = tid; i < E; i += blockDim.x) {
             sja[p0 + i] = ja[p0 + i];
             ssv[p0 + i] = sval[p0 + i];
-            sslot[p0 + i] = 0;  // unused: such tiles are summed from memory
+            if (cbits == 0) sslot[p0 + i] = 0;  // unused: such tiles are summed from memory
         }
         return;
     }
@@ -598,9 +610,18 @@ __global__ __launch_bounds__(256) void k_jac_sort_tiles(const int4* __restrict__
     for (int i = tid; i < E; i += blockDim.x) {
         const unsigned long long kv = key[i];
         const int pos = (int)(kv & 0xffffull);
-        sja[p0 + i] = (int)(kv >> 16);
+        const unsigned col = (unsigned)(kv >> 16);
         ssv[p0 + i] = sval[p0 + pos];
-        sslot[p0 + i] = (unsigned short)pos;
+        if (cbits > 0) {
+            // positions past the packed field belong to tiles the sweep sums
+            // from memory (more entries than its LDS holds): stored as 0
+            const unsigned room = 32u - (unsigned)cbits;
+            const unsigned ps = ((unsigned)pos >> room) ? 0u : (unsigned)pos;
+            sja[p0 + i] = (int)(col | (ps << cbits));
+        } else {
+            sja[p0 + i] = (int)col;
+            sslot[p0 + i] = (unsigned short)pos;
+        }
     }
 }
 
@@ -825,24 +846,41 @@ static void launch_sweep(const TileArgs& AS, int ntiles, const float* rs, const 
             return;
         }
     }
-    if (AS.sja)
-        hipLaunchKernelGGL((k_jac_sweep<K, TW, true>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x,
+    if (AS.sja && AS.cbits > 0)
+        hipLaunchKernelGGL((k_jac_sweep<K, TW, 2>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x,
+                           mode, epl);
+    else if (AS.sja)
+        hipLaunchKernelGGL((k_jac_sweep<K, TW, 1>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x,
                            mode, epl);
     else
-        hipLaunchKernelGGL((k_jac_sweep<K, TW, false>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt,
+        hipLaunchKernelGGL((k_jac_sweep<K, TW, 0>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt,
                            x, mode, epl);
 }
 
-// Column-sorted entry copy of tile table t (allocates; setup only).
-static int ensure_sorted(sp_mat_t* m, TileTable& t) {
+// Column bits of the packed sorted layout for tiles of tw products, or 0
+// when a column and a position the sweep reads (below the tile's LDS
+// capacity) do not fit one 32-bit word together.
+static int packed_column_bits(int n, int tw) {
+    int pbits = 0;
+    while ((1 << pbits) < tw + tw / 4) ++pbits;
+    const int cbits = 32 - pbits;
+    return ((long long)n <= (1ll << cbits)) ? cbits : 0;
+}
+
+// Column-sorted entry copy of tile table t of tiles of tw products
+// (allocates; setup only).  The position of each entry travels in the high
+// bits of its column word when both fit, which saves 2 bytes per entry of
+// every sweep's traffic.
+static int ensure_sorted(sp_mat_t* m, TileTable& t, int tw) {
     MatPriv* pv = priv_of(m);
     if (t.sja || t.count == 0) return SPTRSV_OK;
     const size_t nz = (size_t)std::max(1, m->nnz);
+    t.cbits = packed_column_bits(m->nr, tw);
     SPT_HIP(hipMalloc(&t.sja, sizeof(int) * nz));
     SPT_HIP(hipMalloc(&t.ssv, sizeof(float) * nz));
-    SPT_HIP(hipMalloc(&t.sslot, sizeof(unsigned short) * nz));
+    if (t.cbits == 0) SPT_HIP(hipMalloc(&t.sslot, sizeof(unsigned short) * nz));
     hipLaunchKernelGGL(k_jac_sort_tiles, dim3(t.count), dim3(256), 0, nullptr, reinterpret_cast<const int4*>(t.tiles),
-                       m->ja, pv->jac_sval, t.sja, t.ssv, t.sslot);
+                       m->ja, pv->jac_sval, t.sja, t.ssv, t.sslot, t.cbits);
     SPT_HIP(hipGetLastError());
     SPT_HIP(hipDeviceSynchronize());
     return SPTRSV_OK;
@@ -866,6 +904,7 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
         AS.sja = ts.sja;
         AS.ssv = ts.ssv;
         AS.sslot = ts.sslot;
+        AS.cbits = ts.cbits;
     }
 
     const NormArgs none{nullptr, 1};
@@ -919,7 +958,7 @@ int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, 
     if (c.variant == 1) {
         MatPriv* pv = priv_of(m);
         const int wide = (c.lanes == 2 && pv->jac_tt[1].count > 0) ? 1 : 0;
-        SPT_TRY(ensure_sorted(m, pv->jac_tt[wide]));
+        SPT_TRY(ensure_sorted(m, pv->jac_tt[wide], 1024 << wide));
     }
     switch (pow2_at_least(k)) {
         case 1: return jacobi_schedule<1>(m, c, k, b, x, s);
@@ -946,7 +985,8 @@ int jacobi_values_changed(sp_mat_t* m) {
         for (TileTable& t : pv->jac_tt)
             if (t.sja && t.count > 0) {
                 hipLaunchKernelGGL(k_jac_sort_tiles, dim3(t.count), dim3(256), 0, nullptr,
-                                   reinterpret_cast<const int4*>(t.tiles), m->ja, pv->jac_sval, t.sja, t.ssv, t.sslot);
+                                   reinterpret_cast<const int4*>(t.tiles), m->ja, pv->jac_sval, t.sja, t.ssv, t.sslot,
+                                   t.cbits);
                 SPT_HIP(hipGetLastError());
             }
         SPT_HIP(hipDeviceSynchronize());

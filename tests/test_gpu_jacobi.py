@@ -215,14 +215,17 @@ def test_jacobi_tile_shapes(cfg, param, k):
     d.close()
 
 
-@pytest.mark.parametrize("cfg,param", [(2, 150000), (3, 60000)])
-def test_sorted_tiles_equal_storage_order_bitwise(cfg, param):
+@pytest.mark.parametrize("cfg,param,tiles", [(2, 150000, (1, 2)), (3, 60000, (1, 2)), (2, 1_100_000, (2,))])
+def test_sorted_tiles_equal_storage_order_bitwise(cfg, param, tiles):
     """Column-sorted tiles only reorder the gathers: every product lands in
     its storage slot and rows are summed in storage order, so the solve is
-    bit-identical to the storage-order sweep, for either tile width."""
+    bit-identical to the storage-order sweep, for either tile width.  The
+    sizes cover the position packed into the column word (n <= 2^21 for
+    1024-product tiles, n <= 2^20 for 2048) and the separate position array
+    (2048-product tiles at n > 2^20)."""
     s = I.generate(cfg, param)
     d = Dev(s.ia, s.ja, s.a, analyse=False)
-    for tile in (1, 2):
+    for tile in tiles:
         xs = [d.solve(s.b, lanes=tile, variant=v, window=8, sweeps=12, outer=3, **J) for v in (0, 1)]
         assert np.array_equal(xs[0], xs[1]), tile
     d.close()
