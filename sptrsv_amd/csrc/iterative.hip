@@ -112,14 +112,14 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 }
 
 // Stages one tile in LDS with every independent load issued before the
-// first use: level 1 the tile record; level 2 the row pointers, (with DIAG)
-// the diagonals from their contiguous copy, the per-row right-hand side (V)
-// and the entries; level 3 the gathered iterate; then all LDS stores.  Products a * v[ja]
-// (diagonal included) go to prod when the tile fits.
+// first use: level 1 the tile record; level 2 the row pointers, the per-row
+// right-hand side (V; skipped when rhs is null) and the entries; level 3 the
+// gathered iterate; then all LDS stores.  Products a * v[ja] (diagonal
+// included) go to prod when the tile fits.
 // SORTED: 0 storage order, 1 column-sorted with positions in sslot, 2
 // column-sorted with positions packed above the column's cbits bits (the
 // word is kept whole until the gather needs its column).
-template <int K, int TW, bool DIAG, class V, class P, int SORTED = 0>
+template <int K, int TW, class V, class P, int SORTED = 0>
 struct TileStage {
     static constexpr int W = TW / K;                       // entries per window = max rows per tile
     static constexpr int CAP = tile_cap<TW>();
@@ -130,7 +130,7 @@ struct TileStage {
     bool lds;
 
     __device__ bool run(const TileArgs& A, int tile, const V* rhs, const P* vals, const P* v, int* s_ia,
-                        double* s_dg, V* s_rhs, P* prod) {
+                        V* s_rhs, P* prod) {
         const int tid = threadIdx.x;
         const int k = A.k;
         const int4 tl = A.tiles[tile];
@@ -150,22 +150,16 @@ struct TileStage {
         // the gathers made it wait conservatively and serialised the gathers.
         // level 2
         int iav[RU];
-        double dg[RU];
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
             iav[u] = 1;
-            dg[u] = 1.0;
-            if (u * kTileThreads <= rows) {
-                const int t = min(tid + u * kTileThreads, rows);
-                iav[u] = A.ia[r0 + t];
-                if (DIAG) dg[u] = A.diag[max(r0 + t - 1, 0)];
-            }
+            if (u * kTileThreads <= rows) iav[u] = A.ia[r0 + min(tid + u * kTileThreads, rows)];
         }
         V rv[SU];
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
             rv[u] = V(0);
-            if (u * kTileThreads < rows * K) {
+            if (rhs && u * kTileThreads < rows * K) {
                 const int t = tid + u * kTileThreads;
                 const int lr = min(t / K, rows - 1), rr = min(t & (K - 1), k - 1);
                 rv[u] = rhs[(size_t)(r0 + lr) * k + rr];
@@ -202,12 +196,11 @@ struct TileStage {
         for (int u = 0; u < RU; ++u) {
             const int t = tid + u * kTileThreads;
             if (t <= rows) s_ia[t] = iav[u] - p0;
-            if (DIAG && t >= 1 && t <= rows) s_dg[t - 1] = dg[u];
         }
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
             const int t = tid + u * kTileThreads;
-            if (t < rows * K) s_rhs[t] = (t & (K - 1)) < k ? rv[u] : V(0);
+            if (rhs && t < rows * K) s_rhs[t] = (t & (K - 1)) < k ? rv[u] : V(0);
         }
         if (lds) {
 #pragma unroll
@@ -228,9 +221,9 @@ template <int K, int TW, int SORTED>
 __device__ __forceinline__ void sweep_tile(const TileArgs& A, int tile, const float* __restrict__ rs,
                                            const float* __restrict__ d_old, float* __restrict__ d_new, double* x,
                                            int mode, int epl, float* prod, int* s_ia, float* s_r) {
-    using Stage = TileStage<K, TW, false, float, float, SORTED>;
+    using Stage = TileStage<K, TW, float, float, SORTED>;
     Stage T;
-    if (!T.run(A, tile, rs, A.sval, d_old, s_ia, nullptr, s_r, prod)) return;
+    if (!T.run(A, tile, rs, A.sval, d_old, s_ia, s_r, prod)) return;
     __syncthreads();
     const int tid = threadIdx.x;
     const int k = A.k;
@@ -277,7 +270,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const fl
                                                             const float* __restrict__ d_old,
                                                             float* __restrict__ d_new, double* x, int mode,
                                                             int epl) {
-    using Stage = TileStage<K, TW, false, float, float, SORTED>;
+    using Stage = TileStage<K, TW, float, float, SORTED>;
     __shared__ float prod[Stage::CAP];
     __shared__ int s_ia[Stage::W + 1];
     __shared__ float s_r[TW];
@@ -469,19 +462,20 @@ __device__ __forceinline__ unsigned long long* norm_slot(const NormArgs& N, int 
 
 // r = b - L x in FP64 (diagonal included).  Writes rs = (float)(r / a_ii)
 // when rs_out is set, and with N.norms folds max |r| and max |x| per member.
+// Only the products and row pointers are staged in LDS (14 KB per
+// workgroup, which lets 8 workgroups share a CU); the lane that finishes a
+// row loads its b, a_ii and x from memory before summing the row.
 template <int K>
 __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const double* __restrict__ b,
                                                             const double* __restrict__ x,
                                                             float* __restrict__ rs_out, NormArgs N, int epl) {
     constexpr int TW = 1024;
-    using Stage = TileStage<K, TW, true, double, double>;
+    using Stage = TileStage<K, TW, double, double>;
     __shared__ double prod[Stage::CAP];
     __shared__ int s_ia[Stage::W + 1];
-    __shared__ double s_dg[Stage::W];
-    __shared__ double s_b[TW];
     __shared__ unsigned long long s_nrm[2];
     Stage T;
-    if (!T.run(A, xcd_tile(blockIdx.x, gridDim.x), b, A.a64, x, s_ia, s_dg, s_b, prod)) return;
+    if (!T.run(A, xcd_tile(blockIdx.x, gridDim.x), nullptr, A.a64, x, s_ia, nullptr, prod)) return;
     const int tid = threadIdx.x;
     const int k = A.k;
     if (N.norms && tid < 2) s_nrm[tid] = 0ull;
@@ -497,6 +491,15 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
     for (int ps = 0; ps < npass; ++ps) {
         const int lr = ps * rpp + tid / GL;
         const bool live = lr < T.rows && rr < k;
+        const bool fin = live && e == 0;
+        const size_t i = (size_t)(T.r0 + min(lr, T.rows - 1)) * k + min(rr, k - 1);
+        // the finishing lane's per-row operands, in flight during the sum
+        double bv = 0.0, dv = 1.0, xv = 0.0;
+        if (fin) {
+            bv = b[i];
+            if (rs_out) dv = A.diag[T.r0 + lr];
+            if (N.norms) xv = x[i];
+        }
         double s = 0.0;
         if (live) {
             const int rs0 = s_ia[lr], re = s_ia[lr + 1];
@@ -509,13 +512,12 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
             }
         }
         s = jac_group_sum(s, GL, K);
-        if (live && e == 0) {
-            const size_t i = (size_t)(T.r0 + lr) * k + rr;
-            const double r = s_b[lr * K + rr] - s;
-            if (rs_out) rs_out[i] = (float)(r / s_dg[lr]);
+        if (fin) {
+            const double r = bv - s;
+            if (rs_out) rs_out[i] = (float)(r / dv);
             if (N.norms) {
                 mr = max(mr, (unsigned long long)__double_as_longlong(fabs(r)));
-                mx = max(mx, (unsigned long long)__double_as_longlong(fabs(x[i])));
+                mx = max(mx, (unsigned long long)__double_as_longlong(fabs(xv)));
             }
         }
     }
@@ -1029,6 +1031,28 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
             st = fail(SPTRSV_E_HIP, "read flag");
         return st == SPTRSV_OK && flag == 0;
     };
+    // per-solve time of schedule c: the best of three rounds of `reps`
+    // solves after one warm-up solve (a single round let the choice between
+    // close candidates, such as the two tile widths, flip with clock noise)
+    auto timed = [&](const sptrsv_config_t& c) -> float {
+        st = jacobi_solve(m, c, k, b, x, nullptr);
+        float best_round = 1e30f;
+        for (int round = 0; round < 3 && st == SPTRSV_OK; ++round) {
+            if (hipEventRecord(e0, nullptr) != hipSuccess) {
+                st = fail(SPTRSV_E_HIP, "tuning event");
+                break;
+            }
+            for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
+            float ms = 0;
+            if (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                if (st == SPTRSV_OK) st = fail(SPTRSV_E_HIP, "tuning event");
+                break;
+            }
+            best_round = std::min(best_round, ms / std::max(reps, 1));
+        }
+        return best_round;
+    };
     // point sweeps (variant 0) and, for one right-hand side, block sweeps
     // (variant 2: exact solves of the tiles' diagonal blocks)
     const std::vector<int> point_sweeps{4, 6, 8, 10, 12, 14, 16, 20, 24};
@@ -1057,14 +1081,7 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
                 if (st != SPTRSV_OK) break;
                 continue;
             }
-            st = jacobi_solve(m, c, k, b, x, nullptr);
-            SPT_HIP(hipEventRecord(e0, nullptr));
-            for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
-            SPT_HIP(hipEventRecord(e1, nullptr));
-            SPT_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
-            ms /= std::max(reps, 1);
+            const float ms = timed(c);
             if (st == SPTRSV_OK && ms < bt) {
                 bt = ms;
                 bc = c;
@@ -1074,25 +1091,22 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
     }
     if (st != SPTRSV_OK) break;
     }
-    // column-sorted tiles (point sweeps), then phase-2 entries per lane
-    // (default 8), each on the best schedule so far
-    for (int v : {-1, 2, 4, 16}) {
+    // column-sorted tiles (point sweeps) at either tile width -- the widths
+    // rank differently with sorted entries, and a point sweep's iterates do
+    // not depend on the tiling -- then phase-2 entries per lane (default 8),
+    // each on the best schedule so far
+    const bool point = bc.variant == 0;
+    for (int v : {-1, -2, 2, 4, 16}) {
         if (st != SPTRSV_OK || bt >= 1e30f) break;
         sptrsv_config_t c = bc;
         if (v < 0) {
-            if (bc.variant != 0) continue;
+            if (!point) continue;
             c.variant = 1;  // column-sorted tiles
+            c.lanes = -v;   // tiles of 1024 or 2048 products
         } else {
             c.window = v;
         }
-        st = jacobi_solve(m, c, k, b, x, nullptr);
-        SPT_HIP(hipEventRecord(e0, nullptr));
-        for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
-        SPT_HIP(hipEventRecord(e1, nullptr));
-        SPT_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= std::max(reps, 1);
+        const float ms = timed(c);
         if (st == SPTRSV_OK && ms < bt) {
             bt = ms;
             bc = c;
@@ -1110,13 +1124,7 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
             if (st != SPTRSV_OK) break;
             continue;
         }
-        SPT_HIP(hipEventRecord(e0, nullptr));
-        for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
-        SPT_HIP(hipEventRecord(e1, nullptr));
-        SPT_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= std::max(reps, 1);
+        const float ms = timed(c);
         if (st == SPTRSV_OK && ms < bt) {
             bt = ms;
             bc = c;
@@ -1126,13 +1134,7 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
         sptrsv_config_t c = bc;
         c.sweeps_next = s2;
         if (!certifies(c)) break;
-        SPT_HIP(hipEventRecord(e0, nullptr));
-        for (int r = 0; r < reps && st == SPTRSV_OK; ++r) st = jacobi_solve(m, c, k, b, x, nullptr);
-        SPT_HIP(hipEventRecord(e1, nullptr));
-        SPT_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        SPT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= std::max(reps, 1);
+        const float ms = timed(c);
         if (st == SPTRSV_OK && ms < bt) {
             bt = ms;
             bc.sweeps_next = s2;
