@@ -47,6 +47,7 @@ struct TileTable {
     int* member = nullptr;   // null: one member
     int count = 0;
     int K = 0;               // right-hand-side width the tiles were cut for
+    int rows_max = 0;        // most rows in one tile
     // entries of each tile sorted by column (built on request): column,
     // scaled value, position inside the tile (packed above the column's
     // cbits bits of sja when cbits > 0; else in sslot)

@@ -72,6 +72,7 @@ struct TileArgs {
     const float* ssv;
     const unsigned short* sslot;
     int cbits;
+    int rows_max;  // most rows in one tile of the table (sizes the sweep's LDS)
 };
 
 struct NormArgs {
@@ -264,16 +265,33 @@ __device__ __forceinline__ void sweep_tile(const TileArgs& A, int tile, const fl
 // One Jacobi sweep on the row-scaled system: d_new = rs - N' d_old, with
 // N' = D^-1 N stored as sval and rs = D^-1 r.  mode 0 stores d_new, mode 1
 // sets x = d_new (first refinement step, x was 0), mode 2 adds x += d_new.
-// One workgroup per tile.
+// One workgroup per tile.  Dynamic LDS (sweep_lds_bytes): the products, then
+// row pointers and right-hand sides for the table's longest tile, not for
+// the worst case of TW / K one-entry rows, so typical tables fit more
+// workgroups per CU.
+extern __shared__ float jac_sweep_lds[];
+
+template <int K, int TW>
+__host__ __device__ constexpr int sweep_rows_cap(int rows_max) {
+    return (rows_max > 0 && rows_max < TW / K) ? rows_max : TW / K;
+}
+
+template <int K, int TW>
+size_t sweep_lds_bytes(int rows_max) {
+    const int r = sweep_rows_cap<K, TW>(rows_max);
+    return sizeof(float) * ((size_t)tile_cap<TW>() + (size_t)r + 1 + (size_t)r * K);
+}
+
 template <int K, int TW, int SORTED>
 __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const float* __restrict__ rs,
                                                             const float* __restrict__ d_old,
                                                             float* __restrict__ d_new, double* x, int mode,
                                                             int epl) {
     using Stage = TileStage<K, TW, float, float, SORTED>;
-    __shared__ float prod[Stage::CAP];
-    __shared__ int s_ia[Stage::W + 1];
-    __shared__ float s_r[TW];
+    const int r = sweep_rows_cap<K, TW>(A.rows_max);
+    float* prod = jac_sweep_lds;
+    int* s_ia = reinterpret_cast<int*>(jac_sweep_lds + Stage::CAP);
+    float* s_r = jac_sweep_lds + Stage::CAP + r + 1;
     sweep_tile<K, TW, SORTED>(A, xcd_tile(blockIdx.x, gridDim.x), rs, d_old, d_new, x, mode, epl, prod, s_ia, s_r);
 }
 
@@ -729,6 +747,14 @@ static int pow2_at_least(int k) {
     return K;
 }
 
+// Most rows in one tile of a table of {first row, end row, first entry, end
+// entry} records.
+static int tiles_rows_max(const std::vector<int>& rec) {
+    int r = 0;
+    for (size_t i = 0; i + 3 < rec.size(); i += 4) r = std::max(r, rec[i + 1] - rec[i]);
+    return r;
+}
+
 static int free_table(TileTable& t) {
     for (void* p : {(void*)t.tiles, (void*)t.member, (void*)t.sja, (void*)t.ssv, (void*)t.sslot})
         if (p) SPT_HIP(hipFree(p));
@@ -780,7 +806,9 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
         hipLaunchKernelGGL(k_jac_tiles, dim3(ceil_div(ntiles, 256)), dim3(256), 0, nullptr, m->ia, m->nr,
                            window, ntiles, reinterpret_cast<int4*>(t.tiles));
         SPT_HIP(hipGetLastError());
-        SPT_HIP(hipDeviceSynchronize());
+        std::vector<int> rec(4 * (size_t)ntiles);
+        SPT_HIP(hipMemcpy(rec.data(), t.tiles, sizeof(int) * rec.size(), hipMemcpyDeviceToHost));
+        t.rows_max = tiles_rows_max(rec);
         t.count = ntiles;
         t.K = K;
     }
@@ -832,6 +860,7 @@ int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vec
         SPT_HIP(hipMemcpy(t.tiles, tiles4[w].data(), sizeof(int) * tiles4[w].size(), hipMemcpyHostToDevice));
         SPT_HIP(hipMalloc(&t.member, sizeof(int) * nt));
         SPT_HIP(hipMemcpy(t.member, tile_member[w].data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+        t.rows_max = tiles_rows_max(tiles4[w]);
         t.count = (int)nt;
     }
     return SPTRSV_OK;
@@ -848,14 +877,15 @@ static void launch_sweep(const TileArgs& AS, int ntiles, const float* rs, const 
             return;
         }
     }
+    const size_t lds = sweep_lds_bytes<K, TW>(AS.rows_max);
     if (AS.sja && AS.cbits > 0)
-        hipLaunchKernelGGL((k_jac_sweep<K, TW, 2>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x,
+        hipLaunchKernelGGL((k_jac_sweep<K, TW, 2>), dim3(ntiles), dim3(kTileThreads), lds, s, AS, rs, cur, nxt, x,
                            mode, epl);
     else if (AS.sja)
-        hipLaunchKernelGGL((k_jac_sweep<K, TW, 1>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x,
+        hipLaunchKernelGGL((k_jac_sweep<K, TW, 1>), dim3(ntiles), dim3(kTileThreads), lds, s, AS, rs, cur, nxt, x,
                            mode, epl);
     else
-        hipLaunchKernelGGL((k_jac_sweep<K, TW, 0>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt,
+        hipLaunchKernelGGL((k_jac_sweep<K, TW, 0>), dim3(ntiles), dim3(kTileThreads), lds, s, AS, rs, cur, nxt,
                            x, mode, epl);
 }
 
@@ -900,6 +930,7 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
     TileArgs AS = A;
     AS.tiles = reinterpret_cast<const int4*>(ts.tiles);
     AS.tile_member = ts.member;
+    AS.rows_max = ts.rows_max;
     // block sweeps: in-tile iterations per sweep (-1: point sweeps)
     const int inner = (c.variant == 2 && K == 1) ? (c.inner > 0 ? c.inner : 4) : -1;
     if (c.variant == 1 && ts.sja) {  // column-sorted tiles
