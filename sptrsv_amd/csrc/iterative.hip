@@ -305,7 +305,15 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_sweep(TileArgs A, const fl
 // many inner iterations as the tile has rows it is the exact block solve.
 // Banded and locally coupled systems need a fraction of the point sweeps,
 // and an inner iteration costs LDS traffic only.  A tile too large for LDS
-// takes the point update.
+// takes the point update.  Dynamic LDS (bsweep_lds_bytes): entry values and
+// columns for CAP entries, then row pointers, base and the two iterates for
+// the table's longest tile.
+template <int TW>
+size_t bsweep_lds_bytes(int rows_max) {
+    const int r = sweep_rows_cap<1, TW>(rows_max);
+    return sizeof(float) * (2 * (size_t)tile_cap<TW>() + (size_t)r + 1 + 3 * (size_t)r);
+}
+
 template <int TW>
 __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const float* __restrict__ rs,
                                                              const float* __restrict__ d_old,
@@ -315,11 +323,13 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const f
     constexpr int CAP = tile_cap<TW>();
     constexpr int U = CAP / kTileThreads;
     constexpr int RU = (W + 1 + kTileThreads - 1) / kTileThreads;
-    __shared__ float s_v[CAP];              // off-tile product, or in-tile value
-    __shared__ int s_c[CAP];                // -1 off-tile, else the tile-local column
-    __shared__ int s_ia[W + 1];
-    __shared__ float s_b[W];                // base
-    __shared__ float s_z[2][W];             // inner iterates
+    const int rcap = sweep_rows_cap<1, TW>(A.rows_max);
+    float* const s_v = jac_sweep_lds;                                   // off-tile product, or in-tile value
+    int* const s_c = reinterpret_cast<int*>(jac_sweep_lds + CAP);       // -1 off-tile, else tile-local column
+    int* const s_ia = reinterpret_cast<int*>(jac_sweep_lds + 2 * CAP);  // rcap + 1
+    float* const s_b = jac_sweep_lds + 2 * CAP + rcap + 1;              // base
+    float* const s_z0 = s_b + rcap;                                     // inner iterates
+    float* const s_z1 = s_z0 + rcap;
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int4 tl = A.tiles[tile];
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const f
         if (t <= rows) s_ia[t] = iav[u] - p0;
         if (t < rows) {
             s_b[t] = rv[u];
-            s_z[0][t] = zv[u];
+            s_z0[t] = zv[u];
         }
     }
 #pragma unroll
@@ -429,7 +439,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const f
         const int qrest = q0 + EC * G2;
         for (int it = 0; it < inner; ++it) {
             if (it > 0) __syncthreads();
-            const float* zc = s_z[cur];
+            const float* zc = cur ? s_z1 : s_z0;
             float sm = 0.0f;
 #pragma unroll
             for (int u = 0; u < EC; ++u) sm = __fmaf_rn(cv[u], zc[max(cc[u], 0)], sm);
@@ -438,14 +448,14 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const f
                 if (c >= 0) sm = __fmaf_rn(s_v[q], zc[c], sm);
             }
             sm = jac_group_sum(sm, G2, 1);
-            if (live && e == 0) s_z[cur ^ 1][lr] = s_b[lr] - sm;
+            if (live && e == 0) (cur ? s_z0 : s_z1)[lr] = s_b[lr] - sm;
             cur ^= 1;
         }
     } else {
         for (int it = 0; it < inner; ++it) {
             __syncthreads();
-            const float* zc = s_z[cur];
-            float* zn = s_z[cur ^ 1];
+            const float* zc = cur ? s_z1 : s_z0;
+            float* zn = cur ? s_z0 : s_z1;
             for (int ps = 0; ps < npass; ++ps) {
                 const int lr = ps * rpp + tid / G2;
                 const bool live = lr < rows;
@@ -464,7 +474,7 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const f
         }
     }
     __syncthreads();
-    const float* zf = s_z[cur];
+    const float* zf = cur ? s_z1 : s_z0;
     for (int lr = tid; lr < rows; lr += kTileThreads) {
         const int i = r0 + lr;
         const float d = inner > 0 ? zf[lr] : s_b[lr];
@@ -872,8 +882,8 @@ static void launch_sweep(const TileArgs& AS, int ntiles, const float* rs, const 
                          int mode, int epl, hipStream_t s, int inner = -1) {
     if constexpr (K == 1) {
         if (inner >= 0) {
-            hipLaunchKernelGGL((k_jac_bsweep<TW>), dim3(ntiles), dim3(kTileThreads), 0, s, AS, rs, cur, nxt, x, mode,
-                               epl, inner);
+            hipLaunchKernelGGL((k_jac_bsweep<TW>), dim3(ntiles), dim3(kTileThreads), bsweep_lds_bytes<TW>(AS.rows_max),
+                               s, AS, rs, cur, nxt, x, mode, epl, inner);
             return;
         }
     }
