@@ -141,8 +141,8 @@ struct TileStage {
         E = tl.w - tl.z;
         if (rows <= 0) return false;
         lds = E * K <= CAP;
-        // Load order: the per-row loads (row pointers, diagonal, right-hand
-        // side), guarded by the tile's row count, then the entries and the
+        // Load order: the per-row loads (row pointers, right-hand side),
+        // guarded by the tile's row count, then the entries and the
         // gathers, unconditional with their indices clamped inside the tile
         // (a slot past the tile rereads its last entry and is never stored).
         // With every guarded load issued before the entries, the compiler
