@@ -319,7 +319,9 @@ def main() -> int:
     def run(prec):
         """Timed steps replay the solve captured in a hipGraph (one launch per
         solve); the dominant kernel's duration is then measured live with HIP
-        events around each of its launches, in the same number of eager solves."""
+        events around its launches (each sync-free pass; for Jacobi one pair
+        around each refinement step's consecutive sweeps, divided by their
+        count), in the same number of eager solves."""
         c = tuned[prec]
         for _ in range(args.warmup):
             I.solve_graph(mat, info, prec, db, dx, c, k=k)

@@ -480,8 +480,10 @@ int sptrsv_timer_stop(hipStream_t stream, float* ms);
 int sptrsv_flush_cache(hipStream_t stream);
 
 /* Kernel timing: while enabled, every sync-free solve kernel launch is
- * bracketed by hip events on its own stream.  _read waits for them and
- * returns the summed duration and the number of launches, then clears. */
+ * bracketed by hip events on its own stream, and the Jacobi sweeps of each
+ * refinement step by one pair around the step's consecutive sweeps.  _read
+ * waits for them and returns the summed duration and the number of kernel
+ * launches covered, then clears. */
 int sptrsv_kernel_timer(int enable);
 int sptrsv_kernel_timer_read(float* total_ms, int* launches);
 

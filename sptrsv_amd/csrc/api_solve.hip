@@ -268,12 +268,14 @@ unsigned lds_pad_for(int cu_waves, int wg) {
 static std::mutex g_kt_mu;
 static bool g_kt_on = false;
 static std::vector<hipEvent_t> g_kt_pool;   // pairs
+static std::vector<int> g_kt_launches;      // kernel launches between each pair's events
 static size_t g_kt_used = 0;
 
-// Next free (start, stop) event pair when timing is enabled, else nullptr.
+// Next free (start, stop) event pair when timing is enabled, else nullptr;
+// the caller brackets `launches` consecutive launches of one kernel with it.
 static thread_local bool g_capturing = false;  // no timer events inside a capture
 
-hipEvent_t* kernel_timer_pair(hipStream_t) {
+hipEvent_t* kernel_timer_pair(hipStream_t, int launches) {
     if (g_capturing) return nullptr;
     std::lock_guard<std::mutex> g(g_kt_mu);
     if (!g_kt_on) return nullptr;
@@ -283,8 +285,10 @@ hipEvent_t* kernel_timer_pair(hipStream_t) {
             if (hipEventCreate(&e) != hipSuccess) return nullptr;
             g_kt_pool.push_back(e);
         }
+        g_kt_launches.resize(g_kt_pool.size() / 2, 1);
     }
     hipEvent_t* p = &g_kt_pool[g_kt_used];
+    g_kt_launches[g_kt_used / 2] = std::max(1, launches);
     g_kt_used += 2;
     return p;
 }
@@ -649,14 +653,16 @@ int sptrsv_kernel_timer_read(float* total_ms, int* launches) {
     if (!total_ms || !launches) return fail(SPTRSV_E_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> g(g_kt_mu);
     double tot = 0;
+    int count = 0;
     for (size_t i = 0; i + 1 < g_kt_used; i += 2) {
         SPT_HIP(hipEventSynchronize(g_kt_pool[i + 1]));
         float ms = 0;
         SPT_HIP(hipEventElapsedTime(&ms, g_kt_pool[i], g_kt_pool[i + 1]));
         tot += ms;
+        count += g_kt_launches[i / 2];
     }
     *total_ms = (float)tot;
-    *launches = (int)(g_kt_used / 2);
+    *launches = count;
     g_kt_used = 0;
     return SPTRSV_OK;
 }

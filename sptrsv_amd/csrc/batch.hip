@@ -29,7 +29,6 @@
 
 namespace sptrsv {
 
-hipEvent_t* kernel_timer_pair(hipStream_t s);
 int upload_concat(const int* n, const int* const* ia, const int* const* ja, const double* const* a,
                   int nmat, sp_mat_t* m, std::vector<long long>& row_off);
 int jacobi_solve(sp_mat_t* m, const sptrsv_config_t& c, int k, const double* b, double* x, hipStream_t s);

@@ -115,6 +115,10 @@ void free_mat(sp_mat_t* m);  // device arrays and private state; zeroes *m
 
 // out[i] = sum_{j<i} in[j] on the device (in and out may alias)
 int exclusive_scan(const int* in, int* out, long long n, hipStream_t s);
+// Kernel timer (api_solve.hip): the next (start, stop) event pair while
+// timing is enabled and the stream is not being captured, else nullptr; the
+// caller records it around `launches` consecutive launches of one kernel.
+hipEvent_t* kernel_timer_pair(hipStream_t s, int launches = 1);
 // stable LSD radix sort of (keys, vals) by key in [0, max_key]; tk/tv scratch
 int radix_sort_pairs(int* keys, int* vals, int* tk, int* tv, int n, int max_key, hipStream_t s);
 

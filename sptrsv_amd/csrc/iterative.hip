@@ -876,7 +876,6 @@ int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vec
     return SPTRSV_OK;
 }
 
-hipEvent_t* kernel_timer_pair(hipStream_t s);
 template <int K, int TW>
 static void launch_sweep(const TileArgs& AS, int ntiles, const float* rs, const float* cur, float* nxt, double* x,
                          int mode, int epl, hipStream_t s, int inner = -1) {
@@ -968,19 +967,22 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
         const float* cur = rs;
         float* nxt = pv->jac_a;
         const int sweeps = o == 0 ? sweeps_first : sweeps_later;
+        // the kernel timer brackets the step's consecutive sweeps with one
+        // event pair (events between launches would add their own overhead
+        // to every sweep)
+        hipEvent_t* ev = kernel_timer_pair(s, sweeps - 1);
+        if (ev) SPT_HIP(hipEventRecord(ev[0], s));
         for (int sw = 1; sw < sweeps; ++sw) {
             const bool last = sw == sweeps - 1;
             const int mode = last ? (o == 0 ? 1 : 2) : 0;
-            hipEvent_t* ev = kernel_timer_pair(s);
-            if (ev) SPT_HIP(hipEventRecord(ev[0], s));
             if (wide)
                 launch_sweep<K, 2048>(AS, ts.count, rs, cur, nxt, x, mode, epl, s, inner);
             else
                 launch_sweep<K, 1024>(AS, ts.count, rs, cur, nxt, x, mode, epl, s, inner);
-            if (ev) SPT_HIP(hipEventRecord(ev[1], s));
             cur = nxt;
             nxt = (nxt == pv->jac_a) ? pv->jac_b : pv->jac_a;
         }
+        if (ev) SPT_HIP(hipEventRecord(ev[1], s));
     }
     // certificate: final FP64 residual with per-member max-norms
     const size_t nw = (size_t)with.members * kNormReps * kNormStride;
