@@ -7,7 +7,8 @@
  * CPU fallback.
  *
  * The reference is CUDA C++ and cannot be built or run here (no nvcc, no
- * CUDA headers, no NVIDIA GPU; see the survey), so there is no oracle/_ref.
+ * CUDA headers or libraries such as cuSPARSE and NVML, no NVIDIA GPU), so
+ * there is no oracle/_ref.
  * Each function below restates the reference file:line it cites and is
  * pinned in tests/test_oracle.py as follows:
  *
