@@ -215,6 +215,43 @@ def test_jacobi_tile_shapes(cfg, param, k):
     d.close()
 
 
+def short_row_matrix(n, seed):
+    """Lower CSR whose first 6000 rows are diagonal-only (tiles of exactly
+    TW / K rows, the most a tile can hold) and whose other rows hold at most
+    one off-diagonal entry; diag 1 + |off|."""
+    rng = np.random.default_rng(seed)
+    ia, ja, a = [0], [], []
+    for i in range(n):
+        if i >= 6000 and rng.random() < 0.3:
+            j = int(rng.integers(max(0, i - 3000), i))
+            v = float(rng.uniform(-1, 1))
+            ja += [j, i]
+            a += [v, 1.0 + abs(v)]
+        else:
+            ja.append(i)
+            a.append(float(rng.uniform(1, 2)))
+        ia.append(len(ja))
+    return np.array(ia, np.int32), np.array(ja, np.int32), np.array(a)
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_jacobi_tiles_full_of_one_entry_rows(k):
+    """Tiles holding the most rows a tile can (one entry each) stage their
+    row pointers and right-hand sides in LDS sized for the table's longest
+    tile: every width and entry layout meets the mixed bar."""
+    ia, ja, a = short_row_matrix(40000, k)
+    n = len(ia) - 1
+    rng = np.random.default_rng(10 + k)
+    B = rng.uniform(-1, 1, n * k)
+    ref = O.fwd_f64(ia, ja, a, B, k=k)
+    d = Dev(ia, ja, a, analyse=False)
+    for variant in (0, 1):
+        for tile in (1, 2):
+            x = d.solve(B, k=k, lanes=tile, variant=variant, sweeps=12, outer=3, **J)
+            assert relerr(x, ref) <= 1e-10, (variant, tile)
+    d.close()
+
+
 @pytest.mark.parametrize("cfg,param,tiles", [(2, 150000, (1, 2)), (3, 60000, (1, 2)), (2, 1_100_000, (2,))])
 def test_sorted_tiles_equal_storage_order_bitwise(cfg, param, tiles):
     """Column-sorted tiles only reorder the gathers: every product lands in
