@@ -238,14 +238,15 @@ def short_row_matrix(n, seed):
 def test_jacobi_tiles_full_of_one_entry_rows(k):
     """Tiles holding the most rows a tile can (one entry each) stage their
     row pointers and right-hand sides in LDS sized for the table's longest
-    tile: every width and entry layout meets the mixed bar."""
+    tile: every width, entry layout and (one right-hand side) the block
+    sweep meet the mixed bar."""
     ia, ja, a = short_row_matrix(40000, k)
     n = len(ia) - 1
     rng = np.random.default_rng(10 + k)
     B = rng.uniform(-1, 1, n * k)
     ref = O.fwd_f64(ia, ja, a, B, k=k)
     d = Dev(ia, ja, a, analyse=False)
-    for variant in (0, 1):
+    for variant in ((0, 1, 2) if k == 1 else (0, 1)):   # block sweeps take one right-hand side
         for tile in (1, 2):
             x = d.solve(B, k=k, lanes=tile, variant=variant, sweeps=12, outer=3, **J)
             assert relerr(x, ref) <= 1e-10, (variant, tile)
