@@ -172,6 +172,22 @@ int run_solver(sp_mat_t* gpu_L, dfr_analysis_info_t* info, const double* b,
  * and info (either may be NULL); the structs themselves stay the caller's. */
 void free_device_analysis(sp_mat_t* gpu_L, dfr_analysis_info_t* info);
 
+/* ---- test driver (header/test.h:4) ---------------------------------- */
+
+/* test.h:4 / test.cu:4-30, called by main.cu:259 with the host CSR of the
+ * lower triangle: uploads it, runs the ALL analysis, builds b = L * 1 and runs
+ * the simple, order and multirow solvers once each under BENCH_RUN_SOLVE
+ * timing (common.h:103-124) with the exact x == 1 check (validate_x,
+ * matrix_properties.h:7-29), printing one line per solver like the
+ * reference.  Returns the summed count of x != 1 (0 = all passed) or a
+ * negative status; the reference returns void, so a caller that ignores the
+ * result is unchanged. */
+int test_solve_L_analysis_multirow(const char* filename, int* csrRowPtrL, int* csrColIdxL, double* csrValL,
+                                   int n);
+/* The same for a VALUE_TYPE=float build (FP32 values, the FP32 solvers). */
+int test_solve_L_analysis_multirow_f32(const char* filename, int* csrRowPtrL, int* csrColIdxL, float* csrValL,
+                                       int n);
+
 /* ---- analysis (header/analysis.h:20) --------------------------------- */
 
 /* analysis.h:20 / analysis_csr.cu:272-410.  GPU level-set analysis.  *info
@@ -340,8 +356,7 @@ int sptrsv_batch_choice(const sptrsv_batch_t* bt, int m, sptrsv_config_t* cfg);
  * ones, on the caller's stream; 0 runs them concurrently on two streams.
  * separate 1 solves the sync-free members as one matrix of their own, with
  * its own level analysis and tuned kernel (rows gathered from and scattered
- * back to the batch order); 0 launches them as ticketed tasks.  A persistent
- * tuned kernel for the separate matrix needs serial = 1. */
+ * back to the batch order); 0 launches them as ticketed tasks. */
 int sptrsv_batch_schedule(const sptrsv_batch_t* bt, int* serial, int* separate);
 int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial, int separate);
 /* sptrsv_check for the batch: watchdog, or a Jacobi member that missed its

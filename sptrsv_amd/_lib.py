@@ -87,6 +87,8 @@ _PROTOS = {
     "sptrsv_spsv_solve": (c_int, [c_void_p, c_int, ctypes.c_double, c_void_p, c_void_p, c_int, c_void_p]),
     "sptrsv_spsv_matrix": (POINTER(SpMat), [c_void_p]),
     "sptrsv_spsv_destroy": (None, [c_void_p]),
+    "test_solve_L_analysis_multirow": (c_int, [c_char_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "test_solve_L_analysis_multirow_f32": (c_int, [c_char_p, c_void_p, c_void_p, c_void_p, c_int]),
     "run_analysis": (c_int, [POINTER(AnalysisInfo), POINTER(SpMat)]),
     "run_solver": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int]),
     "free_device_analysis": (None, [POINTER(SpMat), POINTER(AnalysisInfo)]),

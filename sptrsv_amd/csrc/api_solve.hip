@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "syncfree.hpp"
@@ -218,6 +219,7 @@ static int reserve(sp_mat_t* m, int k, bool need_pub, bool need_two_pass) {
     MatPriv* pv = priv_of(m);
     size_t words = (size_t)m->nr * k;
     if (need_pub && pv->pub_words < words) {
+        drop_graphs(pv);
         if (pv->pub) SPT_HIP(hipFree(pv->pub));
         pv->pub = nullptr;
         pv->pub_words = 0;
@@ -225,6 +227,7 @@ static int reserve(sp_mat_t* m, int k, bool need_pub, bool need_two_pass) {
         pv->pub_words = words;
     }
     if (need_two_pass && pv->resid_len < words) {
+        drop_graphs(pv);
         if (pv->b32) SPT_HIP(hipFree(pv->b32));
         if (pv->corr) SPT_HIP(hipFree(pv->corr));
         pv->b32 = nullptr;
@@ -275,19 +278,21 @@ static size_t g_kt_used = 0;
 // the caller brackets `launches` consecutive launches of one kernel with it.
 static thread_local bool g_capturing = false;  // no timer events inside a capture
 
-hipEvent_t* kernel_timer_pair(hipStream_t, int launches) {
-    if (g_capturing) return nullptr;
+KtPair kernel_timer_pair(hipStream_t, int launches) {
+    KtPair p;
+    if (g_capturing) return p;
     std::lock_guard<std::mutex> g(g_kt_mu);
-    if (!g_kt_on) return nullptr;
+    if (!g_kt_on) return p;
     if (g_kt_used + 2 > g_kt_pool.size()) {
         for (int i = 0; i < 64; ++i) {
             hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return p;
             g_kt_pool.push_back(e);
         }
         g_kt_launches.resize(g_kt_pool.size() / 2, 1);
     }
-    hipEvent_t* p = &g_kt_pool[g_kt_used];
+    p.ev[0] = g_kt_pool[g_kt_used];
+    p.ev[1] = g_kt_pool[g_kt_used + 1];
     g_kt_launches[g_kt_used / 2] = std::max(1, launches);
     g_kt_used += 2;
     return p;
@@ -295,9 +300,11 @@ hipEvent_t* kernel_timer_pair(hipStream_t, int launches) {
 
 // ---- one sync-free pass ----------------------------------------------------------------
 
+// timed = false keeps the pass off the kernel timer (whose event pool may
+// grow, which the allocation-free entry point must not do).
 template <class P>
 static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_t& c, int k,
-                    const void* b, void* pub, double* x_out, hipStream_t s) {
+                    const void* b, void* pub, double* x_out, hipStream_t s, bool timed = true) {
     MatPriv* pv = priv_of(m);
     const int K = pow2_ceil(k);
     SolveArgs A{};
@@ -338,7 +345,7 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
         A.persist_grid = (long long)device_cus() * std::max(1, per_cu / std::max(1, c.wg / kWave));
     }
     int u = c.window > 0 ? c.window : window_for(pv, (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW) ? 32 : c.lanes);
-    hipEvent_t* ev = kernel_timer_pair(s);
+    const KtPair ev = timed ? kernel_timer_pair(s) : KtPair{};
     if (ev) SPT_HIP(hipEventRecord(ev[0], s));
     hipError_t e = launch_solve<P, 64>(kind, K, c.wg, u, A, s);
     if (e != hipSuccess) return hip_fail(e, "sync-free solve launch", __FILE__, __LINE__);
@@ -460,6 +467,7 @@ static int solve_oriented(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     const size_t esz = precision == SPTRSV_PREC_F32 ? sizeof(float) : sizeof(double);
     const size_t bytes = std::max<size_t>((size_t)m->nr * k * esz, 8);
     if (pv->rev_bytes < bytes) {
+        drop_graphs(pv);
         if (pv->rev_b) SPT_HIP(hipFree(pv->rev_b));
         if (pv->rev_x) SPT_HIP(hipFree(pv->rev_x));
         pv->rev_b = pv->rev_x = nullptr;
@@ -607,12 +615,14 @@ int sptrsv_check(sp_mat_t* mat, hipStream_t stream) {
 
 int sptrsv_set_backoff(sp_mat_t* mat, int level) {
     if (!mat || !mat->priv || level < 0 || level > 4) return fail(SPTRSV_E_INVALID_ARG, "bad back-off");
+    drop_graphs(priv_of(mat));  // captured launches hold the old setting
     priv_of(mat)->backoff = level;
     return SPTRSV_OK;
 }
 
 int sptrsv_set_lds_pad(sp_mat_t* mat, int bytes) {
     if (!mat || !mat->priv || bytes < 0 || bytes > 65536) return fail(SPTRSV_E_INVALID_ARG, "bad LDS pad");
+    drop_graphs(priv_of(mat));  // captured launches hold the old setting
     priv_of(mat)->lds_pad = (unsigned)bytes;
     return SPTRSV_OK;
 }
@@ -620,6 +630,7 @@ int sptrsv_set_lds_pad(sp_mat_t* mat, int bytes) {
 int sptrsv_trace(sp_mat_t* mat, int k, int enable) {
     if (!mat || !mat->priv || k < 1 || k > kWave) return fail(SPTRSV_E_INVALID_ARG, "bad trace request");
     MatPriv* pv = priv_of(mat);
+    drop_graphs(pv);  // captured launches hold the trace buffer's address
     if (pv->trace) SPT_HIP(hipFree(pv->trace));
     pv->trace = nullptr;
     pv->trace_len = 0;
@@ -670,6 +681,7 @@ int sptrsv_kernel_timer_read(float* total_ms, int* launches) {
 int sptrsv_set_watchdog_ms(sp_mat_t* mat, double ms) {
     if (!mat || !mat->priv || !(ms > 0)) return fail(SPTRSV_E_INVALID_ARG, "bad watchdog bound");
     MatPriv* pv = priv_of(mat);
+    drop_graphs(pv);  // captured launches hold the old bound
     pv->watchdog_ticks = (long long)(ms * pv->wall_rate_khz);
     if (pv->watchdog_ticks < 1) pv->watchdog_ticks = 1;
     return SPTRSV_OK;

@@ -149,6 +149,16 @@ namespace {
 
 constexpr int kBatchWG = 256;
 
+// Destroys the batch's captured solves: they hold the device addresses of
+// the task tables, the separate matrix and the Jacobi workspaces, so this
+// runs before any of them is freed or rebuilt.
+void drop_batch_graphs(sptrsv_batch_t* bt) {
+    if (bt->graphs.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
+    bt->graphs.clear();
+}
+
 int lanes_rule(int avg) {
     if (avg <= 2) return 2;
     if (avg <= 5) return 4;
@@ -196,15 +206,15 @@ int build_tasks(sptrsv_batch_t* bt) {
         matrix_tasks(bt, m, bt->lanes[m], per[m]);
         if (bt->choice[m] != SPTRSV_KERNEL_JACOBI) per_sf[m] = per[m];
     }
+    drop_batch_graphs(bt);
     SPT_TRY(upload_tasks(per, &bt->d_tasks, &bt->ntasks));
     SPT_TRY(upload_tasks(per_sf, &bt->d_tasks_sf, &bt->ntasks_sf));
-    for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
-    bt->graphs.clear();
     return SPTRSV_OK;
 }
 
 // Hands the tiles of the matrices whose choice is JACOBI to the Jacobi path.
 int install_jacobi(sptrsv_batch_t* bt) {
+    drop_batch_graphs(bt);
     const int nmat = (int)bt->choice.size();
     std::vector<int> tiles[2], member[2];
     double delta = 1e300;
@@ -247,7 +257,7 @@ int launch_batch(sptrsv_batch_t* bt, const Task* tasks, int ntasks, const void* 
                        words, P::kUnsolved, pv->counters);
     SPT_HIP(hipGetLastError());
     if (ntasks > 0) {
-        hipEvent_t* ev = kernel_timer_pair(s);
+        const KtPair ev = kernel_timer_pair(s);
         if (ev) SPT_HIP(hipEventRecord(ev[0], s));
         hipLaunchKernelGGL((k_batch<P, kBatchWG>), dim3(ntasks), dim3(kBatchWG), 0, s, A, tasks, ntasks);
         SPT_HIP(hipGetLastError());
@@ -303,6 +313,7 @@ int batch_solve_all(sptrsv_batch_t* bt, int precision, const double* b, double* 
 }
 
 void free_sub(sptrsv_batch_t* bt) {
+    drop_batch_graphs(bt);
     if (bt->sub.priv) free_device_analysis(&bt->sub, bt->sub_info);
     if (bt->sub_info) free(bt->sub_info);
     bt->sub_info = nullptr;
@@ -483,13 +494,9 @@ int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial, int separate) {
         return fail(SPTRSV_E_INVALID_ARG, "bad batch schedule");
     if (separate && !bt->sub.priv)
         return fail(SPTRSV_E_INVALID_ARG, "no separate sync-free matrix: tune the batch (MIXED) first");
-    if (separate && !serial && bt->n_jacobi > 0 && bt->sub_cfg.variant == 2 &&
-        bt->sub_cfg.kernel != SPTRSV_KERNEL_JACOBI)
-        return fail(SPTRSV_E_INVALID_ARG, "a persistent configuration runs only with serial halves");
+    drop_batch_graphs(bt);
     bt->serial = serial != 0;
     bt->use_sub = separate != 0;
-    for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
-    bt->graphs.clear();
     return SPTRSV_OK;
 }
 
@@ -632,9 +639,8 @@ int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
     // The sync-free matrices either through the ticketed task launch or as a
     // matrix of their own (level analysis, tuned kernel); the two halves
     // concurrently on two streams or one after the other (the concurrent
-    // sweeps can starve latency-bound waves).  A persistent configuration of
-    // the separate matrix needs every one of its workgroups resident, so it
-    // only runs one after the other.
+    // sweeps can starve latency-bound waves).  Persistent kernels claim their
+    // work in order, so they progress next to the Jacobi stream too.
     bt->serial = false;
     bt->use_sub = false;
     if (bt->ntasks_sf > 0) {
@@ -648,14 +654,12 @@ int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
                 sub_ok = false;
             }
         }
-        const bool sub_persistent = sub_ok && bt->sub_cfg.variant == 2 && bt->sub_cfg.kernel != SPTRSV_KERNEL_JACOBI;
         float best_t = 1e30f;
         bool best_sub = false, best_serial = false;
         for (int use : {0, 1}) {
             if (use && !sub_ok) continue;
             for (int ser : {0, 1}) {
                 if (bt->n_jacobi == 0 && ser) continue;  // nothing to overlap
-                if (use && sub_persistent && !ser && bt->n_jacobi > 0) continue;
                 bt->use_sub = use != 0;
                 bt->serial = ser != 0;
                 const float t = time_batch(bt, ts.b, ts.x, 5, &st);
@@ -670,8 +674,7 @@ int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
         bt->use_sub = best_sub;
         bt->serial = best_serial;
     }
-    for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
-    bt->graphs.clear();
+    drop_batch_graphs(bt);
     return sptrsv_check(&bt->mat, nullptr);
 }
 
@@ -683,6 +686,9 @@ int sptrsv_batch_tune(sptrsv_batch_t* bt, int precision, int tune) {
     if (!bt) return fail(SPTRSV_E_INVALID_ARG, "null batch");
     if (precision != SPTRSV_PREC_F64 && precision != SPTRSV_PREC_MIXED)
         return fail(SPTRSV_E_INVALID_ARG, "batched solves take FP64 or MIXED precision");
+    // tuning rebuilds task tables, Jacobi tiles and the separate matrix, and
+    // may return early on any path: captured solves go first
+    drop_batch_graphs(bt);
     const int nmat = (int)bt->lanes.size();
     for (int m = 0; m < nmat; ++m) bt->lanes[m] = lanes_rule(bt->avg[m]);
     if (!tune) {
@@ -749,7 +755,7 @@ int sptrsv_batch_check(sptrsv_batch_t* bt, hipStream_t stream) {
 
 void sptrsv_batch_destroy(sptrsv_batch_t* bt) {
     if (!bt) return;
-    for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
+    drop_batch_graphs(bt);
     if (bt->d_tasks) (void)hipFree(bt->d_tasks);
     if (bt->d_sub) (void)hipFree(bt->d_sub);
     if (bt->d_tasks_sf) (void)hipFree(bt->d_tasks_sf);

@@ -95,7 +95,7 @@ struct MatPriv {
     float* jac_a = nullptr;      // Jacobi iterates (ping-pong), n*k each
     float* jac_b = nullptr;
     size_t jac_len = 0;
-    unsigned long long* jac_norms = nullptr;  // per-member max |r|, max |x| (double bits), replicated
+    unsigned long long* jac_norms = nullptr;  // per (member, column) max |r|, max |x| (double bits), replicated
     size_t jac_norms_len = 0;    // in 64-bit words
     TileTable jac_tt[2];         // row tiles of 1024 / 2048 products (residual uses [0])
     bool jac_tiles_fixed = false;  // tiles installed by the caller (batch), never rebuilt
@@ -111,14 +111,25 @@ struct MatPriv {
 
 MatPriv* priv_of(sp_mat_t* m);
 int ensure_priv(sp_mat_t* m);
+// Destroys every captured solve of the matrix.  A captured solve holds the
+// device addresses of the workspaces and the launch arguments it was
+// captured with, so this runs before any of them is freed, reallocated or
+// changed (waits for the device first: a replay may still be running).
+void drop_graphs(MatPriv* pv);
 void free_mat(sp_mat_t* m);  // device arrays and private state; zeroes *m
 
 // out[i] = sum_{j<i} in[j] on the device (in and out may alias)
 int exclusive_scan(const int* in, int* out, long long n, hipStream_t s);
 // Kernel timer (api_solve.hip): the next (start, stop) event pair while
-// timing is enabled and the stream is not being captured, else nullptr; the
-// caller records it around `launches` consecutive launches of one kernel.
-hipEvent_t* kernel_timer_pair(hipStream_t s, int launches = 1);
+// timing is enabled and the stream is not being captured, else an empty
+// pair (false); the caller records it around `launches` consecutive
+// launches of one kernel.  Returned by value: the pool may grow meanwhile.
+struct KtPair {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    explicit operator bool() const { return ev[0] != nullptr; }
+    hipEvent_t operator[](int i) const { return ev[i]; }
+};
+KtPair kernel_timer_pair(hipStream_t s, int launches = 1);
 // stable LSD radix sort of (keys, vals) by key in [0, max_key]; tk/tv scratch
 int radix_sort_pairs(int* keys, int* vals, int* tk, int* tv, int n, int max_key, hipStream_t s);
 

@@ -33,6 +33,13 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 
 MatPriv* priv_of(sp_mat_t* m) { return static_cast<MatPriv*>(m->priv); }
 
+void drop_graphs(MatPriv* pv) {
+    if (!pv || pv->graphs.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (auto& g : pv->graphs) (void)hipGraphExecDestroy(g.second);
+    pv->graphs.clear();
+}
+
 int convert_values_to_f32(sp_mat_t* m);
 void free_info_arrays(dfr_analysis_info_t* info);
 int jacobi_values_changed(sp_mat_t* m);
@@ -423,11 +430,84 @@ int sptrsv_update_values(sp_mat_t* mat, const double* vals, long long count, int
     // the margin into their launches, so they are dropped; a tuned Jacobi
     // choice may no longer apply (dominance can be lost)
     if (mat->a) SPT_TRY(jacobi_values_changed(mat));
-    for (auto& gr : pv->graphs) (void)hipGraphExecDestroy(gr.second);
-    pv->graphs.clear();
+    drop_graphs(pv);
     for (int p = 0; p < 3; ++p)
         if (pv->tuned[p] && pv->tuned_cfg[p].kernel == SPTRSV_KERNEL_JACOBI) pv->tuned[p] = false;
     return SPTRSV_OK;
+}
+
+}  // extern "C"
+
+namespace sptrsv {
+
+// test.h:4 / test.cu:4-30, the flow main.cu:259 runs: upload L, analyse
+// (ALL), b = L * 1 summed in storage order (test.cu:13-16), then each of the
+// three solvers under the BENCH_RUN_SOLVE timing (common.h:103-124, one
+// repetition as BENCH_REPEAT) with the exact x == 1 check (validate_x,
+// matrix_properties.h:7-29), printing one line per solver as the reference
+// does.  V is VALUE_TYPE: double runs the FP64 solvers, float the FP32 ones.
+// Returns the summed validate_x counts (0 = every solver passed), or a
+// negative status.
+template <class V>
+static int test_flow(const char* filename, int* ia, int* ja, V* a, int n) {
+    if (n < 0 || (n > 0 && (!ia || !ja || !a))) return fail(SPTRSV_E_INVALID_ARG, "null CSR arrays");
+    constexpr bool F64 = sizeof(V) == 8;
+    sp_mat_t L{};
+    if constexpr (F64) SPT_TRY(allocate_memory(ia, ja, a, n, &L, ALL));
+    else SPT_TRY(allocate_memory_f32(ia, ja, a, n, &L, ALL));
+    dfr_analysis_info_t info{};
+    info.mode = ALL;
+    int st = run_analysis(&info, &L);
+    std::vector<V> b((size_t)std::max(n, 1), V(0));
+    for (int i = 0; i < n; ++i)
+        for (int j = ia[i]; j < ia[i + 1]; ++j) b[(size_t)i] += a[j];
+    V *db = nullptr, *dx = nullptr;
+    if (st == SPTRSV_OK && hipMalloc(&db, sizeof(V) * b.size()) != hipSuccess) st = fail(SPTRSV_E_ALLOC, "b");
+    if (st == SPTRSV_OK && hipMalloc(&dx, sizeof(V) * b.size()) != hipSuccess) st = fail(SPTRSV_E_ALLOC, "x");
+    if (st == SPTRSV_OK && hipMemcpy(db, b.data(), sizeof(V) * b.size(), hipMemcpyHostToDevice) != hipSuccess)
+        st = fail(SPTRSV_E_HIP, "copy b");
+    if (st == SPTRSV_OK)
+        printf("%s: n = %d, nnz = %d, %d levels\n", filename ? filename : "(matrix)", L.nr, L.nnz, info.nlevs);
+    int errors = 0;
+    const struct {
+        const char* name;
+        int kernel;
+    } solvers[] = {{"solve base", SPTRSV_KERNEL_SIMPLE},
+                   {"solve order", SPTRSV_KERNEL_ORDER},
+                   {"solve multirow", SPTRSV_KERNEL_MULTIROW}};
+    for (const auto& sv : solvers) {
+        if (st != SPTRSV_OK || n == 0) break;
+        sptrsv_config_t c{};
+        c.kernel = sv.kernel;
+        c.fused = 1;
+        sptrsv_bench_t r{};
+        st = sptrsv_bench_solve(&L, &info, F64 ? SPTRSV_PREC_F64 : SPTRSV_PREC_F32, &c, db, dx, n, 1, 1, 1, 1, &r);
+        if (st != SPTRSV_OK) break;
+        printf("%20.20s ----- %6.3f ms (stdev=%6.3f ms) ----- %6.2f W ----- ", sv.name, r.mean_ms, r.stdev_ms,
+               r.power_mw >= 0 ? r.power_mw / 1000.0 : 0.0);
+        if (r.errors) printf("[FAILED! %d errors]\n", r.errors);
+        else printf("[PASS!]\n");
+        errors += r.errors;
+    }
+    fflush(stdout);
+    if (db) (void)hipFree(db);
+    if (dx) (void)hipFree(dx);
+    free_device_analysis(&L, &info);
+    return st != SPTRSV_OK ? st : errors;
+}
+
+}  // namespace sptrsv
+
+extern "C" {
+
+int test_solve_L_analysis_multirow(const char* filename, int* csrRowPtrL, int* csrColIdxL, double* csrValL,
+                                   int n) {
+    return test_flow(filename, csrRowPtrL, csrColIdxL, csrValL, n);
+}
+
+int test_solve_L_analysis_multirow_f32(const char* filename, int* csrRowPtrL, int* csrColIdxL, float* csrValL,
+                                       int n) {
+    return test_flow(filename, csrRowPtrL, csrColIdxL, csrValL, n);
 }
 
 int run_solver(sp_mat_t* gpu_L, dfr_analysis_info_t* info, const double* b, double* x, int n) {

@@ -76,7 +76,9 @@ struct TileArgs {
 };
 
 struct NormArgs {
-    unsigned long long* norms;  // null: no norms; else [member][rep][stride]: [0] |r|, [1] |x|
+    // null: no norms; else one slot per (member, right-hand-side column),
+    // slot = member * k + column, laid out [slot][rep][stride]: [0] |r|, [1] |x|
+    unsigned long long* norms;
     int members;
 };
 
@@ -501,20 +503,20 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
     using Stage = TileStage<K, TW, double, double>;
     __shared__ double prod[Stage::CAP];
     __shared__ int s_ia[Stage::W + 1];
-    __shared__ unsigned long long s_nrm[2];
+    __shared__ unsigned long long s_nrm[2 * K];  // per column: max |r|, max |x|
     Stage T;
     if (!T.run(A, xcd_tile(blockIdx.x, gridDim.x), nullptr, A.a64, x, s_ia, nullptr, prod)) return;
     const int tid = threadIdx.x;
     const int k = A.k;
-    if (N.norms && tid < 2) s_nrm[tid] = 0ull;
+    if (N.norms && tid < 2 * K) s_nrm[tid] = 0ull;
     __syncthreads();
     const int G2 = group_lanes<K>(T.E, T.rows, epl);
     const int GL = G2 * K;
     const int rpp = kTileThreads / GL;
     const int npass = (T.rows + rpp - 1) / rpp;
     const int gl = tid % GL, e = gl / K, rr = gl & (K - 1);
-    // max |r| and max |x| of this thread's rows, as the bit patterns of
-    // non-negative doubles (ordered like the values)
+    // max |r| and max |x| of this thread's rows (all in column rr), as the
+    // bit patterns of non-negative doubles (ordered like the values)
     unsigned long long mr = 0ull, mx = 0ull;
     for (int ps = 0; ps < npass; ++ps) {
         const int lr = ps * rpp + tid / GL;
@@ -550,20 +552,24 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_resid(TileArgs A, const do
         }
     }
     if (N.norms) {
-        // wave maxima first: one LDS atomic per wave instead of one per row
-        // (same-address atomics serialise)
-        for (int o = kWave / 2; o > 0; o >>= 1) {
+        // Each column is certified on its own: a thread's column is
+        // rr = tid & (K - 1), so lanes of one column are K apart.  Wave
+        // maxima per column first (one LDS atomic per wave and column
+        // instead of one per row; same-address atomics serialise), then one
+        // global atomic per column and word.
+        for (int o = kWave / 2; o >= K; o >>= 1) {
             mr = max(mr, (unsigned long long)__shfl_xor((long long)mr, o));
             mx = max(mx, (unsigned long long)__shfl_xor((long long)mx, o));
         }
-        if ((tid & (kWave - 1)) == 0) {
-            if (mr) atomicMax(&s_nrm[0], mr);
-            if (mx) atomicMax(&s_nrm[1], mx);
+        const int wl = tid & (kWave - 1);
+        if (wl < K && wl < k) {  // here rr == wl
+            if (mr) atomicMax(&s_nrm[2 * wl], mr);
+            if (mx) atomicMax(&s_nrm[2 * wl + 1], mx);
         }
         __syncthreads();
-        if (tid < 2 && s_nrm[tid] != 0ull) {
+        if (tid < 2 * K && (tid >> 1) < k && s_nrm[tid] != 0ull) {
             const int m = A.tile_member ? A.tile_member[xcd_tile(blockIdx.x, gridDim.x)] : 0;
-            atomicMax(norm_slot(N, m, tid), s_nrm[tid]);
+            atomicMax(norm_slot(N, m * k + (tid >> 1), tid & 1), s_nrm[tid]);
         }
     }
 }
@@ -679,7 +685,8 @@ __global__ void k_jac_zero_norms(unsigned long long* norms, size_t words) {
         norms[i] = 0ull;
 }
 
-// status = 1 unless, for every member, |r|_inf / delta <= tol * |x|_inf.
+// status = 1 unless, for every slot (member and right-hand-side column),
+// |r|_inf / delta <= tol * |x|_inf.
 __global__ void k_jac_certify(const unsigned long long* norms, int members, double delta, double tol,
                               int* status) {
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < members; m += gridDim.x * blockDim.x) {
@@ -778,6 +785,7 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
     MatPriv* pv = priv_of(m);
     const size_t words = std::max<size_t>((size_t)m->nr * k, 1);
     if (pv->jac_len < words) {
+        drop_graphs(pv);
         for (float** p : {&pv->jac_r, &pv->jac_a, &pv->jac_b})
             if (*p) SPT_HIP(hipFree(*p));
         pv->jac_r = pv->jac_a = pv->jac_b = nullptr;
@@ -787,8 +795,9 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
         SPT_HIP(hipMalloc(&pv->jac_b, words * sizeof(float)));
         pv->jac_len = words;
     }
-    const size_t nw = (size_t)std::max(1, pv->jac_members) * kNormReps * kNormStride;
+    const size_t nw = (size_t)std::max(1, pv->jac_members) * k * kNormReps * kNormStride;
     if (pv->jac_norms_len < nw) {
+        drop_graphs(pv);
         if (pv->jac_norms) SPT_HIP(hipFree(pv->jac_norms));
         pv->jac_norms = nullptr;
         pv->jac_norms_len = 0;
@@ -809,6 +818,7 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
         TileTable& t = pv->jac_tt[w];
         if (t.K == K) continue;
         if (pv->jac_tiles_fixed) return fail(SPTRSV_E_CAPACITY, "batched JACOBI solves take one right-hand side");
+        drop_graphs(pv);
         SPT_TRY(free_table(t));
         const int window = (1024 << w) / K;
         const int ntiles = (int)std::max<long long>(1, ((long long)m->nnz + window - 1) / window);
@@ -856,6 +866,7 @@ void jacobi_host_tiles(const int* ia, int r_begin, int r_end, long long row_base
 int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vector<int>* tile_member,
                      int members, double delta) {
     MatPriv* pv = priv_of(m);
+    drop_graphs(pv);
     pv->jac_tiles_fixed = true;
     pv->jac_members = std::max(1, members);
     pv->margin = delta;
@@ -970,7 +981,7 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
         // the kernel timer brackets the step's consecutive sweeps with one
         // event pair (events between launches would add their own overhead
         // to every sweep)
-        hipEvent_t* ev = kernel_timer_pair(s, sweeps - 1);
+        const KtPair ev = kernel_timer_pair(s, sweeps - 1);
         if (ev) SPT_HIP(hipEventRecord(ev[0], s));
         for (int sw = 1; sw < sweeps; ++sw) {
             const bool last = sw == sweeps - 1;
@@ -984,13 +995,14 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
         }
         if (ev) SPT_HIP(hipEventRecord(ev[1], s));
     }
-    // certificate: final FP64 residual with per-member max-norms
-    const size_t nw = (size_t)with.members * kNormReps * kNormStride;
+    // certificate: final FP64 residual with max-norms per member and column
+    const int slots = with.members * k;
+    const size_t nw = (size_t)slots * kNormReps * kNormStride;
     hipLaunchKernelGGL(k_jac_zero_norms, dim3(ceil_div((long long)nw, 256)), dim3(256), 0, s, pv->jac_norms, nw);
     hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b, (const double*)x,
                        (float*)nullptr, with, epl);
-    hipLaunchKernelGGL(k_jac_certify, dim3(ceil_div(with.members, 256)), dim3(256), 0, s, pv->jac_norms,
-                       with.members, pv->margin, pv->tol, pv->counters + 3);
+    hipLaunchKernelGGL(k_jac_certify, dim3(ceil_div(slots, 256)), dim3(256), 0, s, pv->jac_norms,
+                       slots, pv->margin, pv->tol, pv->counters + 3);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }
@@ -1204,6 +1216,8 @@ extern "C" {
 
 int sptrsv_set_tolerance(sp_mat_t* mat, double rel_tol) {
     if (!mat || !mat->priv || !(rel_tol > 0) || rel_tol >= 1) return fail(SPTRSV_E_INVALID_ARG, "bad tolerance");
+    // captured solves certify against the tolerance they were captured with
+    drop_graphs(priv_of(mat));
     priv_of(mat)->tol = rel_tol;
     return SPTRSV_OK;
 }

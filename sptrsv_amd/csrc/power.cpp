@@ -6,7 +6,12 @@
 // instead of busy-polling; the GPU is matched by PCI address to the HIP
 // device; errors are returned instead of exit(0) (nvmlPower.cpp:71-137).
 // The reading is amdsmi_power_info_t::current_socket_power (W), the MI300+
-// field; the running mean is returned in mW like nvmlAPIEnd.
+// field, else average_socket_power; amdsmi sets fields it does not support
+// to UINT32_MAX, and such a field (or 0) is no reading -- a sample without a
+// reading is not counted.  The running mean is returned in mW like
+// nvmlAPIEnd.  Like nvmlAPIRun / nvmlAPIEnd (nvmlPower.cpp:68, 150), each
+// run initialises amdsmi and each end shuts it down; a run still going when
+// the library is unloaded is stopped and shut down then.
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -41,6 +46,15 @@ typedef int (*fn_procs)(smi_handle, uint32_t*, smi_handle*);
 typedef int (*fn_power)(smi_handle, smi_power_info*);
 typedef int (*fn_bdf)(smi_handle, uint64_t*);
 constexpr uint64_t kInitGpus = 1ull << 1;
+constexpr uint32_t kUnsupported = 0xFFFFFFFFu;  // amdsmi: field not supported
+
+// Socket power in W from one amdsmi reading, or a negative value when the
+// reading holds none (fields unsupported or zero).
+double watts_of(const smi_power_info& q) {
+    if (q.current_socket_power != 0 && q.current_socket_power != kUnsupported) return q.current_socket_power;
+    if (q.average_socket_power != 0 && q.average_socket_power != kUnsupported) return q.average_socket_power;
+    return -1.0;
+}
 
 struct Sampler {
     std::mutex mu;
@@ -110,6 +124,24 @@ smi_handle find_gpu(Sampler& s, int device) {
     return first;  // single-GPU fallback
 }
 
+// Stops the sampling thread and shuts amdsmi down (caller holds s.mu).
+void stop_locked(Sampler& s) {
+    s.run = false;
+    if (s.th.joinable()) s.th.join();
+    if (s.inited) {
+        (void)s.shut();
+        s.inited = false;
+    }
+    s.gpu = nullptr;
+}
+
+// Library teardown: a sampler left running would otherwise hold a joinable
+// thread and an initialised amdsmi into process exit.
+__attribute__((destructor)) void power_teardown() {
+    std::lock_guard<std::mutex> g(g_s.mu);
+    stop_locked(g_s);
+}
+
 }  // namespace
 
 using namespace sptrsv;
@@ -122,14 +154,15 @@ int amdsmiAPIRun(const char* label, int device, int period_us) {
     std::lock_guard<std::mutex> g(s.mu);
     if (s.run.load()) return fail(SPTRSV_E_INVALID_ARG, "power sampling already running");
     if (!load(s)) return fail(SPTRSV_E_UNAVAILABLE, "amdsmi library not loadable");
-    if (!s.inited) {
-        if (s.init(kInitGpus) != 0) return fail(SPTRSV_E_UNAVAILABLE, "amdsmi_init failed");
-        s.inited = true;
-    }
+    if (s.init(kInitGpus) != 0) return fail(SPTRSV_E_UNAVAILABLE, "amdsmi_init failed");
+    s.inited = true;
     s.gpu = find_gpu(s, device);
-    if (!s.gpu) return fail(SPTRSV_E_UNAVAILABLE, "no amdsmi GPU handle");
     smi_power_info pi{};
-    if (s.power(s.gpu, &pi) != 0) return fail(SPTRSV_E_UNAVAILABLE, "amdsmi_get_power_info failed");
+    if (!s.gpu || s.power(s.gpu, &pi) != 0 || watts_of(pi) < 0) {
+        const char* why = !s.gpu ? "no amdsmi GPU handle" : "amdsmi reports no socket power for this GPU";
+        stop_locked(s);
+        return fail(SPTRSV_E_UNAVAILABLE, why);
+    }
     s.mean_w = 0.0;
     s.samples = 0;
     s.run = true;
@@ -138,10 +171,11 @@ int amdsmiAPIRun(const char* label, int device, int period_us) {
         while (s.run.load()) {
             smi_power_info q{};
             if (s.power(s.gpu, &q) == 0) {
-                double w = q.current_socket_power ? (double)q.current_socket_power
-                                                  : (double)q.average_socket_power;
-                ++s.samples;
-                s.mean_w += (w - s.mean_w) / s.samples;  // running mean, nvmlPower.cpp:52
+                const double w = watts_of(q);
+                if (w >= 0) {
+                    ++s.samples;
+                    s.mean_w += (w - s.mean_w) / s.samples;  // running mean, nvmlPower.cpp:52
+                }
             }
             std::this_thread::sleep_for(std::chrono::microseconds(period));
         }
@@ -153,8 +187,7 @@ float amdsmiAPIEnd(void) {
     Sampler& s = g_s;
     std::lock_guard<std::mutex> g(s.mu);
     if (!s.run.load()) return -1.0f;
-    s.run = false;
-    if (s.th.joinable()) s.th.join();
+    stop_locked(s);
     return s.samples ? (float)(s.mean_w * 1000.0) : -1.0f;
 }
 

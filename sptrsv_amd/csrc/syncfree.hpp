@@ -741,19 +741,34 @@ __global__ __launch_bounds__(WG) void k_fast_multirow(SolveArgs A) {
 }
 
 // ---- persistent lean path --------------------------------------------------------
-// Exactly gridDim.x*WG/64 waves, launched cooperatively so all are resident;
-// wave g owns work items g, g+W, g+2W, ... and processes them in increasing
-// order (MODE 0: rows in natural order, 1: rows in iorder, 2: multirow waves
-// of the analysis).  No dispatch per row and no ticket atomics; W sets how
-// many waves poll at once.  Deadlock freedom: the lowest unfinished item is
-// at the head of its wave's queue and every item it depends on is lower.
+// A fixed number of workgroups (the residency cap) walks the work items:
+// rows in natural order (MODE 0), rows in iorder (MODE 1) or multirow waves of
+// the analysis (MODE 2).  Items are claimed, one per wave, from the ticket
+// counter in increasing order, and each wave works its claims in the order it
+// made them; it claims its next item as it starts the current one, so the
+// claim's round trip overlaps the current item's polls.
+//
+// Deadlock freedom, whatever the number of resident waves (other kernels may
+// hold part of the device): every claimed item is held by a wave that is
+// running.  Let u be the lowest unfinished item; it has been claimed, since a
+// wave only claims after finishing everything it claimed before, and every
+// item it depends on is lower, hence finished.  If u is the current item of
+// its wave, that wave completes it.  If u is its wave's pre-claimed next
+// item, the wave's current item is lower than u, hence finished, and the wave
+// moves on to u.  Either way u completes, so every item does.  A wave that
+// gets an item past the end stops; the kernel ends when every wave has.
+__device__ __forceinline__ int claim_issue(int* ctr, int lane) {
+    return lane == 0 ? atomicAdd(ctr, 1) : 0;
+}
+__device__ __forceinline__ int claim_take(int raw) { return __builtin_amdgcn_readlane(raw, 0); }
+
 template <class P, int WG, int U, int MODE>
 __global__ __launch_bounds__(WG) void k_persist(SolveArgs A) {
-    const int W = gridDim.x * (WG / kWave);
-    const int wave = blockIdx.x * (WG / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
     const int count = (MODE == 2) ? A.n_waves : A.n;
-    for (int t = wave; t < count; t += W) {
+    int t = claim_take(claim_issue(A.ticket, lane));
+    while (t < count) {
+        const int next = claim_issue(A.ticket, lane);
         if constexpr (MODE == 2) {
             const int vs = A.wsize[t];
             const int L = vs == 0 ? 1 : vs;
@@ -765,24 +780,24 @@ __global__ __launch_bounds__(WG) void k_persist(SolveArgs A) {
             const int row = (MODE == 1) ? A.iorder[t] : t;
             fast_rows<P, U>(A, A.L, lane, lane < A.L ? row : -1);
         }
+        t = claim_take(next);
     }
 }
 
-// Persistent general engine (any K): the same resident-wave walk for lane
-// groups of L entry lanes x K right-hand-side lanes.  An item is the 64/(L*K)
-// rows one wave holds, consecutive in natural order (MODE 0) or in iorder
-// (MODE 1), or one multirow wave of the analysis (MODE 2, rows of one level,
-// in passes of 64/(L*K) rows).  Rows of an item may depend on each other in
-// MODE 0/1; sf_rows publishes inside its polling loop, so that is safe.  The
-// deadlock argument is k_persist's: items are walked in increasing order and
-// every dependency of the lowest unfinished item is lower or inside it.
+// Persistent general engine (any K): the same claimed walk for lane groups
+// of L entry lanes x K right-hand-side lanes.  An item is the 64/(L*K) rows
+// one wave holds, consecutive in natural order (MODE 0) or in iorder (MODE
+// 1), or one multirow wave of the analysis (MODE 2, rows of one level, in
+// passes of 64/(L*K) rows).  Rows of an item may depend on each other in
+// MODE 0/1; sf_rows publishes inside its polling loop, so that is safe, and
+// the deadlock argument is k_persist's.
 template <class P, int K, int WG, int U, int MODE>
 __global__ __launch_bounds__(WG) void k_persist_rows(SolveArgs A) {
-    const int W = gridDim.x * (WG / kWave);
-    const int wave = blockIdx.x * (WG / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
     if constexpr (MODE == 2) {
-        for (int t = wave; t < A.n_waves; t += W) {
+        int t = claim_take(claim_issue(A.ticket, lane));
+        while (t < A.n_waves) {
+            const int next = claim_issue(A.ticket, lane);
             const int vs = A.wsize[t];
             int L = vs == 0 ? 1 : vs;
             if (L * K > kWave) L = kWave / K;
@@ -794,15 +809,19 @@ __global__ __launch_bounds__(WG) void k_persist_rows(SolveArgs A) {
                 const int row = (c0 + seg < nrows) ? A.iorder[b0 + c0 + seg] : -1;
                 sf_rows<P, K, U>(A, L, lane, row, nullptr, 0, 0);
             }
+            t = claim_take(next);
         }
     } else {
         const int lgG = A.lgL + kLog2<K>();
         const int rpw = kWave >> lgG;
         const int items = (A.n + rpw - 1) / rpw;
-        for (int t = wave; t < items; t += W) {
+        int t = claim_take(claim_issue(A.ticket, lane));
+        while (t < items) {
+            const int next = claim_issue(A.ticket, lane);
             const int pos = t * rpw + (lane >> lgG);
             const int row = pos < A.n ? (MODE == 1 ? A.iorder[pos] : pos) : -1;
             sf_rows<P, K, U>(A, A.L, lane, row, nullptr, 0, 0);
+            t = claim_take(next);
         }
     }
 }
@@ -828,36 +847,31 @@ enum KernelKind {
     KK_PERSIST_MULTIROW = 9,  // persistent walk over multirow waves (lean for K == 1)
 };
 
-// Cooperative launch of a persistent kernel with A.persist_grid workgroups,
-// clamped to what can be resident at once.
-template <class P, int WG, int U, int MODE>
-hipError_t launch_persist(const SolveArgs& A, hipStream_t s) {
-    const void* f = reinterpret_cast<const void*>(&k_persist<P, WG, U, MODE>);
+// Launch of a persistent kernel with A.persist_grid workgroups, clamped to
+// what can be resident at once (more would only start as others finish).  An
+// ordinary launch: progress does not depend on all workgroups being resident.
+template <class F>
+hipError_t launch_persistent(F* kern, int wg, const SolveArgs& A, hipStream_t s) {
+    const void* f = reinterpret_cast<const void*>(kern);
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, A.dyn_lds);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, wg, A.dyn_lds);
     if (e != hipSuccess) return e;
-    const long long cap = (long long)per_cu * cus;
-    unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(A.persist_grid, cap));
-    SolveArgs a = A;
-    void* args[] = {&a};
-    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(WG), args, A.dyn_lds, s);
+    const long long cap = (long long)std::max(1, per_cu) * cus;
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(A.persist_grid, cap));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(wg), A.dyn_lds, s, A);
+    return hipGetLastError();
+}
+
+template <class P, int WG, int U, int MODE>
+hipError_t launch_persist(const SolveArgs& A, hipStream_t s) {
+    return launch_persistent(&k_persist<P, WG, U, MODE>, WG, A, s);
 }
 
 template <class P, int K, int WG, int U, int MODE>
 hipError_t launch_persist_rows(const SolveArgs& A, hipStream_t s) {
-    const void* f = reinterpret_cast<const void*>(&k_persist_rows<P, K, WG, U, MODE>);
-    int dev = 0, cus = 0, per_cu = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, A.dyn_lds);
-    if (e != hipSuccess) return e;
-    const long long cap = (long long)per_cu * cus;
-    unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(A.persist_grid, cap));
-    SolveArgs a = A;
-    void* args[] = {&a};
-    return hipLaunchCooperativeKernel(f, dim3(grid), dim3(WG), args, A.dyn_lds, s);
+    return launch_persistent(&k_persist_rows<P, K, WG, U, MODE>, WG, A, s);
 }
 
 template <class P, int K, int WG, int U>
