@@ -295,6 +295,24 @@ int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision,
 int csr_L_solve_mixed(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b,
                       double* x, int n, int k, int refine_steps, hipStream_t stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the fused mixed-precision solve of sptrsv_solve
+ * (SPTRSV_PREC_MIXED, fused = 1, refine_steps = 1), with its published
+ * words in a fixed per-device scratch array of SPTRSV_NOALLOC_MAX_WORDS
+ * 64-bit words (32 MiB, held by every GPU the library runs on until the
+ * process ends) instead of the matrix's workspace.  It calls no hipMalloc,
+ * hipFree or other allocating routine.  n * k above the maximum is
+ * SPTRSV_E_CAPACITY.  Sync-free kernels only: a configuration that resolves
+ * to JACOBI is SPTRSV_E_INVALID_ARG, as is a handle stored in reversed order
+ * (allocate_memory_upper, or the transpose of a lower-triangular handle).
+ * Each call re-initialises the scratch it uses.  Every call on one GPU
+ * shares that GPU's scratch, so calls that overlap in time on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+#define SPTRSV_NOALLOC_MAX_WORDS (1 << 22)
+int sptrsv_solve_mixed_noalloc(sp_mat_t* mat, dfr_analysis_info_t* info, const sptrsv_config_t* cfg,
+                               const double* b, double* x, int n, int k, hipStream_t stream);
+
 /* Allocates (outside any capture) the workspaces the solves above use for k
  * right-hand sides, so later calls allocate nothing. */
 int sptrsv_reserve(sp_mat_t* mat, int k);

@@ -31,6 +31,7 @@ SIMPLE, ORDER, MULTIROW, ALL = 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_ORDER, KERNEL_MULTIROW, KERNEL_JACOBI = 0, 1, 2, 3, 4
 E_NOT_CONVERGED = -11
 PREC_F64, PREC_F32, PREC_MIXED = 0, 1, 2
+NOALLOC_MAX_WORDS = 1 << 22  # SPTRSV_NOALLOC_MAX_WORDS
 
 
 class SptrsvUnavailable(ImportError):
@@ -105,6 +106,8 @@ _PROTOS = {
                                    c_void_p, c_int, c_int, c_int, c_void_p]),
     "csr_L_solve_mixed": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int, c_int,
                                   c_int, c_void_p]),
+    "sptrsv_solve_mixed_noalloc": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), POINTER(Config), c_void_p,
+                                           c_void_p, c_int, c_int, c_void_p]),
     "sptrsv_reserve": (c_int, [POINTER(SpMat), c_int]),
     "sptrsv_check": (c_int, [POINTER(SpMat), c_void_p]),
     "sptrsv_set_watchdog_ms": (c_int, [POINTER(SpMat), c_double]),

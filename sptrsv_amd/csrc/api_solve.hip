@@ -490,6 +490,35 @@ int solve_checked(sp_mat_t* m, dfr_analysis_info_t* info, int precision, const s
     return solve_oriented(m, info, precision, c, b, x, k, refine_steps, s);
 }
 
+// ---- allocation-free fused mixed solve ----------------------------------------------
+// The published words of sptrsv_solve_mixed_noalloc live in this fixed array
+// (one copy per GPU, held until the process ends) instead of the matrix's
+// workspace.  Host code never passes the array's name to a kernel: its
+// device address is looked up once per device with hipGetSymbolAddress.
+static __device__ __align__(256) uint64_t scratch[SPTRSV_NOALLOC_MAX_WORDS];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint64_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (looked up on the
+// first call on that device, with it current).
+static int scratch_on_current_device(uint64_t** out) {
+    int dev = 0;
+    SPT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(SPTRSV_E_CAPACITY, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        if (!p) return fail(SPTRSV_E_HIP, "scratch array has no device address");
+        g_scratch_ptr[dev] = static_cast<uint64_t*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SPTRSV_OK;
+}
+
 }  // namespace sptrsv
 
 using namespace sptrsv;
@@ -584,6 +613,27 @@ int csr_L_solve_multirow_f32(sp_mat_t* mat, dfr_analysis_info_t* info, const flo
 int csr_L_solve_mixed(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b, double* x, int n,
                       int k, int refine_steps, hipStream_t stream) {
     return solve_checked(mat, info, SPTRSV_PREC_MIXED, nullptr, b, x, n, k, refine_steps, stream);
+}
+
+int sptrsv_solve_mixed_noalloc(sp_mat_t* mat, dfr_analysis_info_t* info, const sptrsv_config_t* cfg,
+                               const double* b, double* x, int n, int k, hipStream_t stream) {
+    SPT_TRY(check_solve_args(mat, SPTRSV_PREC_MIXED, b, x, n, k, 1));
+    if ((long long)n * k > (long long)SPTRSV_NOALLOC_MAX_WORDS)
+        return fail(SPTRSV_E_CAPACITY, "n * k = " + std::to_string((long long)n * k) +
+                                           " exceeds the allocation-free scratch of " +
+                                           std::to_string((long long)SPTRSV_NOALLOC_MAX_WORDS) + " words");
+    if (priv_of(mat)->reversed)
+        return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve takes lower-triangular handles only");
+    if (n == 0) return SPTRSV_OK;
+    sptrsv_config_t c;
+    SPT_TRY(resolve_config(mat, info, SPTRSV_PREC_MIXED, k, cfg, &c));
+    if (c.kernel == SPTRSV_KERNEL_JACOBI)
+        return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve runs the sync-free kernels only "
+                                          "(JACOBI keeps per-matrix workspaces); name SIMPLE, ORDER or MULTIROW");
+    c.fused = 1;
+    uint64_t* pub = nullptr;
+    SPT_TRY(scratch_on_current_device(&pub));
+    return run_pass<PolMixed>(mat, info, c, k, b, pub, x, stream, false);
 }
 
 int sptrsv_reserve(sp_mat_t* mat, int k) {

@@ -271,6 +271,16 @@ def solve_graph(mat, info, precision: int, b: DeviceArray, x: DeviceArray, cfg: 
           "sptrsv_solve_graph")
 
 
+def solve_mixed_noalloc(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, cfg: Config | None = None,
+                        stream=None) -> None:
+    """The fused mixed solve with its published words in the library's fixed
+    per-device scratch (sptrsv_solve_mixed_noalloc): allocates nothing, n*k <=
+    NOALLOC_MAX_WORDS, sync-free kernels only, overlapping calls on one GPU race."""
+    check(L.lib().sptrsv_solve_mixed_noalloc(byref(mat), byref(info) if info is not None else None,
+                                             byref(cfg) if cfg is not None else None, b.ptr, x.ptr, mat.nr, k,
+                                             stream), "sptrsv_solve_mixed_noalloc")
+
+
 def reserve(mat: SpMat, k: int) -> None:
     check(L.lib().sptrsv_reserve(byref(mat), k), "sptrsv_reserve")
 
