@@ -93,6 +93,11 @@ def bytes_jacobi_method(n, nnz, k, sweeps, outer, sweeps_next=0):
 
 KERNEL_NAMES = {1: "simple", 2: "order", 3: "multirow", 4: "jacobi"}
 
+# Configurations the autotuner chose on MI355X for each workload (sptrsv_config_t
+# fields), used with --preset so that profiler runs time exactly the tuned
+# kernels without running the tuner.
+PRESETS = {}
+
 
 def cfg_dict(c) -> dict:
     d = {"kernel": KERNEL_NAMES.get(c.kernel, c.kernel), "lanes": c.lanes, "wg": c.wg}
@@ -104,7 +109,14 @@ def cfg_dict(c) -> dict:
     else:
         d.update(fused=c.fused, variant={0: "multi-row-wave", 1: "one-row-per-wave", 2: "persistent"}.get(
             c.variant, c.variant), cu_waves=c.cu_waves, window=c.window)
+        if c.variant == 2:
+            d["claim_rounds"] = c.claim_rounds or 1
     return d
+
+
+def cfg_fields(c) -> dict:
+    """Every sptrsv_config_t field (the form PRESETS records)."""
+    return {name: getattr(c, name) for name, _ in c._fields_}
 
 
 def sweep_layout(n, tw, variant):
@@ -132,21 +144,75 @@ def profiled_traffic(kernel_key: str):
     return None
 
 
-def cpu_baseline(s, budget_s: float):
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def _native_substitution():
+    """The oracle's serial FP64 forward substitution compiled on this host with
+    gcc -O3 -march=native (the CPU baseline's build), or None without gcc."""
+    import ctypes
+    import subprocess
+    import tempfile
+    out = Path(tempfile.mkdtemp(prefix="sptrsv_cpu_")) / "liboracle_native.so"
+    try:
+        subprocess.run(["gcc", "-O3", "-march=native", "-fPIC", "-shared", "-ffp-contract=off",
+                        str(ROOT / "oracle" / "oracle.c"), "-o", str(out)], check=True, capture_output=True,
+                       timeout=120)
+        return ctypes.CDLL(str(out))
+    except (OSError, subprocess.SubprocessError):
+        return None
+
+
+def cpu_baseline(systems, budget_s: float):
+    """Serial FP64 forward substitution, one core, median of >= 5 timed solves
+    after a warm-up, over the given systems in turn until the time budget is
+    spent; GNNZ/s over the solves timed.  Built with -O3 -march=native on this
+    host; its results are checked bit for bit against the oracle."""
+    import ctypes
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as O
-    O.fwd_f64(s.ia, s.ja, s.a, s.b[: s.n])               # warm-up
-    times = []
+    lib = _native_substitution()
+    flags = "gcc -O3 -march=native -ffp-contract=off, built on this host" if lib else \
+        "gcc -O2 -ffp-contract=off (the checker's build; no compiler on this host)"
+
+    def p(a):
+        return a.ctypes.data_as(ctypes.c_void_p)
+
+    def solve(sy):
+        if lib is None:
+            return O.fwd_f64(sy.ia, sy.ja, sy.a, sy.b[: sy.n])
+        x = np.zeros(sy.n)
+        b = np.ascontiguousarray(sy.b[: sy.n])
+        lib.oracle_fwd_f64(sy.n, p(sy.ia), p(sy.ja), p(sy.a), p(b), p(x), 1)
+        return x
+    first = systems[0]
+    x = solve(first)                                        # warm-up, and the bitwise check
+    assert np.array_equal(x, O.fwd_f64(first.ia, first.ja, first.a, first.b[: first.n]))
+    rates, secs, solves, entries = [], 0.0, 0, 0
     t_end = time.perf_counter() + budget_s
-    while time.perf_counter() < t_end or len(times) < 5:
+    i = 0
+    while time.perf_counter() < t_end or len(rates) < 5:
+        sy = systems[i % len(systems)]
+        i += 1
         t0 = time.perf_counter()
-        O.fwd_f64(s.ia, s.ja, s.a, s.b[: s.n])
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return {"value": round(s.nnz / med / 1e9, 4), "unit": "GNNZ/s", "cores": 1, "kind": "port",
-            "sample": f"serial FP64 forward substitution (oracle/oracle.c, gcc -O2) on the full matrix, "
-                      f"1 RHS, median of {len(times)} solves ({sum(times):.1f} s), host has "
-                      f"{os.cpu_count()} cores, 1 used"}
+        solve(sy)
+        dt = time.perf_counter() - t0
+        rates.append(sy.nnz / dt)
+        secs += dt
+        solves += 1
+        entries += sy.nnz
+    what = "the full matrix" if len(systems) == 1 else f"{min(solves, len(systems))} of the {len(systems)} systems"
+    return {"value": round(float(np.median(rates)) / 1e9, 4), "unit": "GNNZ/s", "cores": 1, "kind": "port",
+            "sample": f"serial FP64 forward substitution (oracle/oracle.c, {flags}) on {what}, 1 RHS, "
+                      f"median of {solves} solves ({secs:.1f} s, {entries / 1e9:.2f} G entries) on "
+                      f"{_cpu_model()}, 1 of {os.cpu_count()} cores used"}
 
 
 def run_batch(args, I, O) -> int:
@@ -172,16 +238,27 @@ def run_batch(args, I, O) -> int:
     for _ in range(args.warmup):
         bt.solve(I.PREC_MIXED, db, dx, graph=True)
     I.synchronize()
-    I.kernel_timer(True)
     timer = I.Timer()
     timer.start()
     for _ in range(args.steps):
         bt.solve(I.PREC_MIXED, db, dx, graph=True)
     msm = timer.stop() / args.steps
-    kms, launches = I.kernel_timer_read()
-    I.kernel_timer(False)
     I.synchronize()
     assert bt.check() == 0
+    # the two halves' kernels timed apart (HIP events on their own streams, in
+    # as many eager solves): Jacobi sweeps of the Jacobi members, and the
+    # sync-free pass of the others
+    per_kind = {}
+    for kind in ("sweeps", "sync-free"):
+        I.kernel_timer(True, kind)
+        for _ in range(args.steps):
+            bt.solve(I.PREC_MIXED, db, dx, graph=False)
+        I.synchronize()
+        per_kind[kind] = I.kernel_timer_read()
+        I.kernel_timer(False)
+    assert bt.check() == 0
+    kms = sum(v[0] for v in per_kind.values())
+    launches = sum(v[1] for v in per_kind.values())
     x = dx.numpy()
     errs = []
     for m, sm in enumerate(systems):
@@ -217,6 +294,27 @@ def run_batch(args, I, O) -> int:
         xx.free()
     log(f"[bench] FP64 batched one launch: {ms64b:.3f} ms; FP64 sequential: {ms_seq:.1f} ms")
     bytes_jac_rows = sum(sm.nnz for m, sm in enumerate(systems) if kinds[m] == I.KERNEL_JACOBI)
+    # roofline of the dominant kernel: the kind with the larger summed time
+    jac_n = sum(sm.n for m, sm in enumerate(systems) if kinds[m] == I.KERNEL_JACOBI)
+    sf_n, sf_nnz = rows - jac_n, nnz - bytes_jac_rows
+    t_sw, n_sw = per_kind["sweeps"]
+    t_sf, n_sf = per_kind["sync-free"]
+    if jc is not None and t_sw >= t_sf and n_sw > 0:
+        b_launch = bytes_jacobi_sweeps(jac_n, bytes_jac_rows, 1, jc.sweeps, jc.outer, jc.sweeps_next)
+        k_ms = t_sw / n_sw
+        kname = (f"Jacobi {'block ' if jc.variant == 2 else ''}sweep (FP32) over the {kinds.count(I.KERNEL_JACOBI)} "
+                 f"Jacobi members, inside FP64 refinement")
+        note = ("effective bandwidth: the Jacobi members' arrays stay in the 256 MB Infinity Cache across a "
+                "step's sweeps when they fit")
+    else:
+        b_launch = bytes_mixed_fused(sf_n, sf_nnz, 1)
+        k_ms = t_sf / max(n_sf, 1)
+        kname = "sync-free fused mixed-precision pass over the sync-free members"
+        note = "one sync-free pass, HBM-fed; latency-bound by the members' dependency chains"
+    achieved = b_launch / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    log(f"[bench] kernels per solve: sweeps {t_sw / args.steps:.3f} ms ({n_sw // max(args.steps, 1)} launches), "
+        f"sync-free {t_sf / args.steps:.3f} ms ({n_sf // max(args.steps, 1)} launches)")
+    cpu = None if args.no_cpu else cpu_baseline(systems, args.cpu_seconds)
     out = {
         "metric": METRIC,
         "value": round(nnz / (msm * 1e-3) / 1e9, 4),
@@ -238,10 +336,24 @@ def run_batch(args, I, O) -> int:
             "f64_sequential_ms": round(ms_seq, 2),
             "speedup_vs_sequential_f64": round(ms_seq / msm, 2),
             "speedup_vs_batched_f64": round(ms64b / msm, 3),
-            "kernel_ms_sum_per_solve": round(kms / max(args.steps, 1), 4), "kernel_launches": launches,
+            "kernel_ms_sum_per_solve": round(kms / max(args.steps, 1), 4),
+            "kernel_launches_per_solve": launches // max(args.steps, 1),
+            "sweeps_ms_per_solve": round(t_sw / max(args.steps, 1), 4),
+            "sync_free_ms_per_solve": round(t_sf / max(args.steps, 1), 4),
         },
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "kernel": kname,
+            "kernel_ms": round(k_ms, 4),
+            "bytes_per_launch": int(b_launch),
+            "note": note,
+        },
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     db.free()
@@ -265,6 +377,8 @@ def main() -> int:
                          "with this schedule (tile width 1 or 2 = 1024 or 2048 products, phase-2 entries per "
                          "lane, 1 = column-sorted tiles, sweeps of the steps after the first) instead of "
                          "tuning it")
+    ap.add_argument("--preset", action="store_true",
+                    help="use the recorded tuned configurations (PRESETS) instead of tuning (profiler runs)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the rocSPARSE, power and cold-cache legs (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -308,6 +422,8 @@ def main() -> int:
                 ap.error("--jacobi takes 5 or 6 integers")
             tuned[prec] = I._cfg(I.KERNEL_JACOBI, j[2], 256, 1, j[4], 0, j[0], j[1], j[3],
                                  j[5] if len(j) == 6 else 0)
+        elif args.preset and args.workload in PRESETS:
+            tuned[prec] = I._cfg(**PRESETS[args.workload]["f64" if prec == I.PREC_F64 else "mixed"])
         elif args.no_tune:
             tuned[prec] = I.default_config(mat, info, prec, k)
         else:
@@ -358,6 +474,29 @@ def main() -> int:
             out.append(timer.stop())
         return float(np.median(out))
 
+    def jacobi_setup_ms():
+        """One-time set-up of the Jacobi path on a fresh handle of the same matrix
+        (row-scaled FP32 copy, contiguous diagonal, tile tables, column-sorted
+        tile copy): its first solve minus a steady eager solve."""
+        c = tuned[I.PREC_MIXED]
+        if c.kernel != I.KERNEL_JACOBI:
+            return None
+        m2 = I.allocate_memory(s.ia, s.ja, s.a)
+        I.synchronize()
+        timer = I.Timer()
+        timer.start()
+        I.reserve(m2, k)
+        I.solve(m2, None, I.PREC_MIXED, db, dx, k=k, cfg=c)
+        first = timer.stop()
+        timer = I.Timer()
+        timer.start()
+        for _ in range(3):
+            I.solve(m2, None, I.PREC_MIXED, db, dx, k=k, cfg=c)
+        steady = timer.stop() / 3
+        assert I.check_watchdog(m2) == 0
+        I.free_device_analysis(m2, None)
+        return round(max(0.0, first - steady), 3)
+
     ms64, kms64, err64 = run(I.PREC_F64)
     log(f"[bench] plain FP64: {ms64:.3f} ms/solve (kernel {kms64:.3f} ms) rel err {err64:.2e}")
     msm, kmsm, errm = run(I.PREC_MIXED)
@@ -367,8 +506,12 @@ def main() -> int:
         # the plain FP64 sync-free solve one right-hand side at a time (k solves)
         b1 = I.DeviceArray.from_numpy(np.ascontiguousarray(s.b.reshape(s.n, k)[:, 0]))
         x1 = I.DeviceArray(s.n, np.float64)
-        c1, _ = (I.default_config(mat, info, I.PREC_F64, 1), 0.0) if args.no_tune else \
-            I.autotune(mat, info, I.PREC_F64, k=1, reps=3)
+        if args.preset and args.workload in PRESETS:
+            c1 = I._cfg(**PRESETS[args.workload]["f64_per_rhs"])
+        elif args.no_tune:
+            c1 = I.default_config(mat, info, I.PREC_F64, 1)
+        else:
+            c1, _ = I.autotune(mat, info, I.PREC_F64, k=1, reps=3)
         for _ in range(args.warmup):
             I.solve_graph(mat, info, I.PREC_F64, b1, x1, c1, k=1)
         I.synchronize()
@@ -387,6 +530,7 @@ def main() -> int:
         b1.free()
         x1.free()
     cold64, coldm = (cold(I.PREC_F64), cold(I.PREC_MIXED)) if not args.no_extras else (float("nan"),) * 2
+    jac_setup = jacobi_setup_ms()
     tdep = I.dep_latency_ns(2000)
     log(f"[bench] cold-cache: FP64 {cold64:.3f} ms, mixed {coldm:.3f} ms; t_dep {tdep:.0f} ns one-way")
 
@@ -457,6 +601,7 @@ def main() -> int:
         power = round(p / 1000.0, 1) if p > 0 else None
 
     n, nnz = s.n, s.nnz
+    cm_is_jacobi = tuned[I.PREC_MIXED].kernel == I.KERNEL_JACOBI
     gnnz_m = k * nnz / (msm * 1e-3) / 1e9
     gnnz_64 = k * nnz / (ms64 * 1e-3) / 1e9
     cm = tuned[I.PREC_MIXED]
@@ -475,7 +620,7 @@ def main() -> int:
         b_launch = b_method = bytes_mixed_fused(n, nnz, k)
         kname = "sync-free fused mixed-precision solve"
     achieved = b_launch / (kmsm * 1e-3) / 1e9
-    cpu = None if args.no_cpu else cpu_baseline(s, args.cpu_seconds)
+    cpu = None if args.no_cpu else cpu_baseline([s], args.cpu_seconds)
     out = {
         "metric": METRIC,
         "value": round(gnnz_m, 4),
@@ -497,18 +642,22 @@ def main() -> int:
             "plain_f64_gnnz_s": round(gnnz_64, 4), "plain_f64_ms": round(ms64, 4),
             "speedup_mixed_vs_f64": round(ms64 / msm, 3),
             "hbm_gbs_mixed": round(b_method / (msm * 1e-3) / 1e9, 2),
+            "hbm_gbs_mixed_compulsory": round(bytes_method_convention(n, nnz, k) / (msm * 1e-3) / 1e9, 2),
             "bytes_mixed_method": int(b_method),
             "hbm_gbs_f64": round(bytes_f64(n, nnz, k) / (ms64 * 1e-3) / 1e9, 2),
             "bytes_method_separate_pass_convention": bytes_method_convention(n, nnz, k),
             "cold_cache_ms": None if args.no_extras else {"mixed": round(coldm, 4), "f64": round(cold64, 4)},
             "t_dep_ns": round(tdep, 1),
-            "latency_bound_fraction_mixed": round(nlevs * tdep * 1e-6 / msm, 3),
+            "latency_bound_fraction_mixed": (None if cm_is_jacobi else round(nlevs * tdep * 1e-6 / msm, 3)),
             "latency_bound_fraction_f64": round(nlevs * tdep * 1e-6 / ms64, 3),
             "power_w": power,
             "rocsparse_csrsv": rocsparse,
             "rocsparse_csritsv": itsv,
             "analysis_ms": {"first": round(analysis_ms[0], 3), "repeat": round(analysis_ms[1], 3)},
+            "jacobi_setup_ms": jac_setup,
             "plain_f64_per_rhs": per_rhs,
+            "tuned_fields": {"f64": cfg_fields(tuned[I.PREC_F64]), "mixed": cfg_fields(tuned[I.PREC_MIXED]),
+                             **({"f64_per_rhs": cfg_fields(c1)} if k > 1 else {})},
         },
         "roofline": {
             "bound": "hbm",
@@ -520,6 +669,10 @@ def main() -> int:
             "kernel": kname,
             "kernel_ms": round(kmsm, 4),
             "bytes_per_launch": int(b_launch),
+            "note": (f"effective bandwidth: the {b_launch / 1e6:.0f} MB a sweep reads stay in the 256 MB "
+                     f"Infinity Cache across a step's sweeps, so achieved counts cache-fed bytes; a whole "
+                     f"cold-cache solve takes {coldm:.3f} ms against {msm:.3f} ms warm"
+                     if cm_is_jacobi else "one sync-free pass, HBM-fed"),
         },
         "cpu_baseline": cpu,
     }

@@ -196,6 +196,24 @@ int test_solve_L_analysis_multirow_f32(const char* filename, int* csrRowPtrL, in
  * stops after iorder; MULTIROW/ALL also build the warp arrays. */
 int multirow_analysis_base_GPU(dfr_analysis_info_t** info, sp_mat_t* gpu_L, MODE mode);
 
+/* Analysis reuse: no counterpart in the reference, which recomputes the
+ * analysis (analysis_csr.cu:272-410 into dfr_syncfree.h:25-41) on every run.
+ * sptrsv_save_analysis writes the analysis of mat to a file: magic, mode and
+ * sizes, an FNV-1a checksum of mat's structure (ia, ja), an FNV-1a checksum
+ * of the whole content, then level, iorder, inv_iorder, lev_size, bucket_off
+ * and (MULTIROW / ALL) warp_lev and both warp layouts.
+ * sptrsv_load_analysis reads such a file into *info (the caller's struct; an
+ * analysis it held is freed first) for a matrix of the same structure:
+ * SPTRSV_E_IO for a damaged or truncated file, SPTRSV_E_INVALID_ARG for a
+ * file of another structure.  sptrsv_copy_analysis gives dst_mat, which must
+ * have the structure of src_mat (checked), its own copy of src's analysis,
+ * e.g. for another system on the same pattern.  Free with
+ * free_device_analysis as usual. */
+int sptrsv_save_analysis(const char* path, sp_mat_t* mat, dfr_analysis_info_t* info);
+int sptrsv_load_analysis(const char* path, sp_mat_t* mat, dfr_analysis_info_t* info);
+int sptrsv_copy_analysis(sp_mat_t* src_mat, dfr_analysis_info_t* src, sp_mat_t* dst_mat,
+                         dfr_analysis_info_t* dst);
+
 /* ---- solvers (header/solver.h:5-8) ----------------------------------- */
 /* b and x are device pointers.  x is fully overwritten.  All solves are
  * stream-ordered, allocate nothing after the first call for a given matrix
@@ -267,6 +285,9 @@ typedef struct {
     int inner;       /* JACOBI with variant 2 (block sweeps, one right-hand
                         side): Jacobi iterations on each tile's diagonal
                         block per sweep, in LDS (0 = 4) */
+    int claim_rounds; /* persistent variant (2): a workgroup claims blocks of
+                         waves * claim_rounds work items with one atomic, a
+                         wave taking claim_rounds of them in turn (0 = 1) */
 } sptrsv_config_t;
 
 /* Relative accuracy the JACOBI path certifies (default 1e-10). */
@@ -388,8 +409,9 @@ int csr_L_rocsparse_create(sp_mat_t* mat, sptrsv_rocsparse_t** out, float* analy
 int csr_L_solve_rocsparse(sptrsv_rocsparse_t* h, const double* b, double* x, hipStream_t stream);
 /* rocSPARSE csritsv (FP64 Jacobi with a residual stopping test, host
  * driven) as a second baseline: x starts at 0; iterates until
- * |b - L x|_inf <= tol or max_iter iterations (free_iter iterations between
- * two tests); *iters gets the iterations done. */
+ * |b - L x|_inf <= tol or max_iter stopping tests (free_iter untested
+ * iterations before each test); *iters gets the Jacobi iterations performed,
+ * (k + 1) * free_iter + k for the k rocSPARSE reports. */
 int csr_L_rocsparse_itsv_create(sp_mat_t* mat, sptrsv_rocsparse_t** out, float* analysis_ms);
 int csr_L_solve_rocsparse_itsv(sptrsv_rocsparse_t* h, const double* b, double* x, double tol,
                                int max_iter, int free_iter, int* iters, hipStream_t stream);
@@ -514,7 +536,8 @@ int sptrsv_flush_cache(hipStream_t stream);
 
 /* Kernel timing: while enabled, every sync-free solve kernel launch is
  * bracketed by hip events on its own stream, and the Jacobi sweeps of each
- * refinement step by one pair around the step's consecutive sweeps.  _read
+ * refinement step by one pair around the step's consecutive sweeps.  enable:
+ * 0 off, 1 both, 2 sync-free launches only, 3 Jacobi sweeps only.  _read
  * waits for them and returns the summed duration and the number of kernel
  * launches covered, then clears. */
 int sptrsv_kernel_timer(int enable);

@@ -16,8 +16,10 @@
  *                       systems; the exact x == 1 known answer of
  *                       test.cu:13-16 / matrix_properties.h:14 on the 2D
  *                       5-point Laplacian (4/-1 values) and on unit-valued L.
- *   oracle_fwd_f32      exact x == 1 known answer in FP32; numpy float32
- *                       row loop on small systems.
+ *   oracle_fwd_f32      exact x == 1 known answer in FP32; bit for bit
+ *                       against a numpy float32 row loop (every product,
+ *                       difference and the reciprocal rounded to float32)
+ *                       on random systems.
  *   oracle_levels       closed forms lev = x+y+1 (2D 5-point, nlevs 127 at
  *                       64x64) and x+y+z+1 (3D 7-point, nlevs 382 at 128^3);
  *                       Kahn peeling and memoised DFS longest path in Python
@@ -91,12 +93,14 @@ void oracle_fwd_f32(int n, const int* ia, const int* ja, const float* a,
                     const float* b, float* x, int k) {
     for (int i = 0; i < n; ++i) {
         int s = ia[i], e = ia[i + 1];
-        float piv = 1.0f / a[e - 1];
+        int dpos = (g_mutation == 1) ? s : e - 1;
+        float piv = 1.0f / a[dpos];
         for (int r = 0; r < k; ++r) {
             float left = b[(size_t)i * k + r];
             for (int p = s; p < e - 1; ++p) {
-                float prod = a[p] * x[(size_t)ja[p] * k + r];
-                left = left - prod;
+                int pp = (g_mutation == 1) ? p + 1 : p;
+                float prod = a[pp] * x[(size_t)ja[pp] * k + r];
+                if (g_mutation == 2) left = left + prod; else left = left - prod;
             }
             x[(size_t)i * k + r] = left * piv;
         }

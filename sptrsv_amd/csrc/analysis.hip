@@ -17,8 +17,12 @@
 //   6. warps       ceil(count*size/lanes) per bucket, exclusive scan, per-warp
 //                  base row and size, sentinel ibase_row[n_warps] = n
 //                  (:354-385, kernel_base :243-270)
+#include <stdio.h>
+#include <string.h>
+
 #include <algorithm>
 #include <mutex>
+#include <string>
 #include <unordered_set>
 #include <vector>
 
@@ -523,11 +527,165 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
     return SPTRSV_OK;
 }
 
+// ---- analysis files and copies ------------------------------------------------------
+unsigned long long fnv1a(const void* p, size_t n, unsigned long long h);
+constexpr unsigned long long kFnvBasis = 0xcbf29ce484222325ULL;
+const char kAnalysisMagic[8] = {'S', 'P', 'T', 'R', 'A', 'N', 'L', '1'};
+
+// The device arrays of an analysis in file order, with their lengths.
+struct InfoArray {
+    int** dev;
+    long long len;
+};
+static std::vector<InfoArray> info_arrays(dfr_analysis_info_t* info, bool warps) {
+    const long long n = info->n, nl = info->nlevs;
+    std::vector<InfoArray> a{{&info->level, n},       {&info->iorder, n},           {&info->inv_iorder, n},
+                             {&info->lev_size, nl},   {&info->bucket_off, n ? 7 * nl + 1 : 0}};
+    if (warps) {
+        a.push_back({&info->warp_lev, nl + 1});
+        a.push_back({&info->ibase_row, (long long)info->n_warps + 1});
+        a.push_back({&info->ivect_size_warp, info->n_warps});
+        a.push_back({&info->ibase_row64, (long long)info->n_waves + 1});
+        a.push_back({&info->ivect_size_wave, info->n_waves});
+    }
+    return a;
+}
+
+// FNV-1a of a matrix's structure (row pointers and column indices, copied
+// from the device).
+static int structure_hash(sp_mat_t* m, unsigned long long* h) {
+    std::vector<int> ia((size_t)m->nr + 1), ja((size_t)std::max(m->nnz, 0));
+    SPT_HIP(hipMemcpy(ia.data(), m->ia, sizeof(int) * ia.size(), hipMemcpyDeviceToHost));
+    if (!ja.empty()) SPT_HIP(hipMemcpy(ja.data(), m->ja, sizeof(int) * ja.size(), hipMemcpyDeviceToHost));
+    unsigned long long g = fnv1a(ia.data(), sizeof(int) * ia.size(), kFnvBasis);
+    *h = fnv1a(ja.data(), sizeof(int) * ja.size(), g);
+    return SPTRSV_OK;
+}
+
+static bool has_warps(const dfr_analysis_info_t* info) { return info->mode == MULTIROW || info->mode == ALL; }
+
 }  // namespace sptrsv
 
 using namespace sptrsv;
 
 extern "C" {
+
+int sptrsv_save_analysis(const char* path, sp_mat_t* mat, dfr_analysis_info_t* info) {
+    if (!path || !mat || !mat->priv || !info) return fail(SPTRSV_E_INVALID_ARG, "null argument");
+    if (!owns_info(info)) return fail(SPTRSV_E_NOT_ANALYSED, "info holds no analysis of this library");
+    if (info->n != mat->nr) return fail(SPTRSV_E_INVALID_ARG, "the analysis belongs to a matrix of another size");
+    const bool warps = has_warps(info);
+    long long head[8] = {(long long)info->mode, info->n, mat->nnz, info->nlevs, info->n_warps, info->n_waves, 0, 0};
+    unsigned long long sh = 0;
+    SPT_TRY(structure_hash(mat, &sh));
+    std::vector<std::vector<int>> host;
+    unsigned long long h = fnv1a(head, sizeof(head), kFnvBasis);
+    h = fnv1a(&sh, sizeof(sh), h);
+    for (const InfoArray& a : info_arrays(info, warps)) {
+        host.emplace_back((size_t)a.len);
+        if (a.len > 0) SPT_HIP(hipMemcpy(host.back().data(), *a.dev, sizeof(int) * (size_t)a.len, hipMemcpyDeviceToHost));
+        h = fnv1a(host.back().data(), sizeof(int) * host.back().size(), h);
+    }
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(SPTRSV_E_IO, std::string("cannot create ") + path);
+    bool ok = fwrite(kAnalysisMagic, 1, 8, f) == 8 && fwrite(head, sizeof(head), 1, f) == 1 &&
+              fwrite(&sh, 8, 1, f) == 1 && fwrite(&h, 8, 1, f) == 1;
+    for (const auto& v : host)
+        if (ok && !v.empty()) ok = fwrite(v.data(), sizeof(int), v.size(), f) == v.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? SPTRSV_OK : fail(SPTRSV_E_IO, std::string("write error on ") + path);
+}
+
+int sptrsv_load_analysis(const char* path, sp_mat_t* mat, dfr_analysis_info_t* info) {
+    if (!path || !mat || !mat->priv || !info) return fail(SPTRSV_E_INVALID_ARG, "null argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(SPTRSV_E_IO, std::string("cannot open ") + path);
+    char magic[8];
+    long long head[8];
+    unsigned long long sh = 0, h = 0;
+    const bool hdr = fread(magic, 1, 8, f) == 8 && memcmp(magic, kAnalysisMagic, 8) == 0 &&
+                     fread(head, sizeof(head), 1, f) == 1 && fread(&sh, 8, 1, f) == 1 && fread(&h, 8, 1, f) == 1;
+    const long long mode = hdr ? head[0] : -1, n = hdr ? head[1] : -1, nlevs = hdr ? head[3] : -1;
+    if (!hdr || mode < ORDER || mode > ALL || n < 0 || n > 0x7ffffffeLL || nlevs < 0 || nlevs > n ||
+        head[4] < 0 || head[5] < 0 || head[4] > 0x7ffffffeLL || head[5] > 0x7ffffffeLL) {
+        fclose(f);
+        return fail(SPTRSV_E_IO, std::string("not an analysis file: ") + path);
+    }
+    if (n != mat->nr || head[2] != mat->nnz) {
+        fclose(f);
+        return fail(SPTRSV_E_INVALID_ARG, "the analysis file belongs to a matrix of another size");
+    }
+    dfr_analysis_info_t t{};
+    t.mode = (MODE)mode;
+    t.n = (int)n;
+    t.nlevs = t.lev_ctr = (int)nlevs;
+    t.n_warps = (int)head[4];
+    t.n_waves = (int)head[5];
+    const bool warps = has_warps(&t);
+    std::vector<std::vector<int>> host;
+    unsigned long long g = fnv1a(head, sizeof(head), kFnvBasis);
+    g = fnv1a(&sh, sizeof(sh), g);
+    bool ok = true;
+    for (const InfoArray& a : info_arrays(&t, warps)) {
+        host.emplace_back((size_t)a.len);
+        if (ok && a.len > 0) ok = fread(host.back().data(), sizeof(int), (size_t)a.len, f) == (size_t)a.len;
+        if (ok) g = fnv1a(host.back().data(), sizeof(int) * host.back().size(), g);
+    }
+    fclose(f);
+    if (!ok || g != h) return fail(SPTRSV_E_IO, std::string(ok ? "checksum mismatch in " : "truncated analysis file ") + path);
+    unsigned long long mh = 0;
+    SPT_TRY(structure_hash(mat, &mh));
+    if (mh != sh) return fail(SPTRSV_E_INVALID_ARG, "the analysis file belongs to a matrix of another structure");
+    // install: the caller's previous arrays go first
+    free_info_arrays(info);
+    *info = t;
+    info->priv = new_info_priv();
+    int st = SPTRSV_OK;
+    if (hipMalloc(&info->row_ctr, sizeof(int)) != hipSuccess || hipMemset(info->row_ctr, 0, sizeof(int)) != hipSuccess)
+        st = fail(SPTRSV_E_ALLOC, "row_ctr");
+    size_t i = 0;
+    for (const InfoArray& a : info_arrays(info, warps)) {
+        const std::vector<int>& v = host[i++];
+        if (st != SPTRSV_OK || a.len == 0) continue;
+        if (hipMalloc(a.dev, sizeof(int) * v.size()) != hipSuccess ||
+            hipMemcpy(*a.dev, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+            st = fail(SPTRSV_E_ALLOC, "analysis arrays");
+    }
+    if (st != SPTRSV_OK) free_info_arrays(info);
+    return st;
+}
+
+int sptrsv_copy_analysis(sp_mat_t* src_mat, dfr_analysis_info_t* src, sp_mat_t* dst_mat, dfr_analysis_info_t* dst) {
+    if (!src_mat || !src_mat->priv || !dst_mat || !dst_mat->priv || !src || !dst || src == dst)
+        return fail(SPTRSV_E_INVALID_ARG, "bad arguments");
+    if (!owns_info(src)) return fail(SPTRSV_E_NOT_ANALYSED, "src holds no analysis of this library");
+    if (src_mat->nr != dst_mat->nr || src_mat->nnz != dst_mat->nnz || src->n != src_mat->nr)
+        return fail(SPTRSV_E_INVALID_ARG, "the matrices differ in size");
+    unsigned long long hs = 0, hd = 0;
+    SPT_TRY(structure_hash(src_mat, &hs));
+    SPT_TRY(structure_hash(dst_mat, &hd));
+    if (hs != hd) return fail(SPTRSV_E_INVALID_ARG, "the matrices differ in structure");
+    const bool warps = has_warps(src);
+    dfr_analysis_info_t t = *src;
+    for (const InfoArray& a : info_arrays(&t, warps)) *a.dev = nullptr;
+    t.row_ctr = nullptr;
+    free_info_arrays(dst);
+    *dst = t;
+    dst->priv = new_info_priv();
+    int st = SPTRSV_OK;
+    if (hipMalloc(&dst->row_ctr, sizeof(int)) != hipSuccess || hipMemset(dst->row_ctr, 0, sizeof(int)) != hipSuccess)
+        st = fail(SPTRSV_E_ALLOC, "row_ctr");
+    const std::vector<InfoArray> from = info_arrays(src, warps), to = info_arrays(dst, warps);
+    for (size_t i = 0; i < to.size() && st == SPTRSV_OK; ++i) {
+        if (to[i].len == 0) continue;
+        const size_t bytes = sizeof(int) * (size_t)to[i].len;
+        if (hipMalloc(to[i].dev, bytes) != hipSuccess ||
+            hipMemcpy(*to[i].dev, *from[i].dev, bytes, hipMemcpyDeviceToDevice) != hipSuccess)
+            st = fail(SPTRSV_E_ALLOC, "analysis arrays");
+    }
+    if (st != SPTRSV_OK) free_info_arrays(dst);
+    return st;
+}
 
 int multirow_analysis_base_GPU(dfr_analysis_info_t** info, sp_mat_t* gpu_L, MODE mode) {
     if (!info || !gpu_L || !gpu_L->priv) return fail(SPTRSV_E_INVALID_ARG, "null info or matrix");

@@ -188,11 +188,13 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     if (out->inner < 0 || out->inner > 64) return fail(SPTRSV_E_INVALID_ARG, "inner must be 0..64");
     if (out->wg != 64 && out->wg != 256 && out->wg != 1024) out->wg = 256;
     if (K > 1) {
-        out->wg = 256;
-        // several RHS: the general engine, ticketed or persistent
+        // several RHS: the general engine, ticketed (256 threads) or
+        // persistent (256 or 1024 threads)
         if (out->kernel != SPTRSV_KERNEL_JACOBI && out->variant != 2) out->variant = 0;
+        if (!(out->variant == 2 && out->wg == 1024)) out->wg = 256;
     }
     if (out->cu_waves < 0) out->cu_waves = 0;
+    if (out->claim_rounds < 0 || out->claim_rounds > 64) return fail(SPTRSV_E_INVALID_ARG, "claim_rounds must be 0..64");
     if (out->kernel == SPTRSV_KERNEL_JACOBI) {
         if (out->window < 0 || out->window > 64) return fail(SPTRSV_E_INVALID_ARG, "window must be 0..64");
     } else {
@@ -269,7 +271,7 @@ unsigned lds_pad_for(int cu_waves, int wg) {
 
 // ---- kernel timer ---------------------------------------------------------------------
 static std::mutex g_kt_mu;
-static bool g_kt_on = false;
+static int g_kt_on = 0;  // 0 off, 1 every timed launch, 2 sync-free passes only, 3 Jacobi sweeps only
 static std::vector<hipEvent_t> g_kt_pool;   // pairs
 static std::vector<int> g_kt_launches;      // kernel launches between each pair's events
 static size_t g_kt_used = 0;
@@ -278,11 +280,19 @@ static size_t g_kt_used = 0;
 // the caller brackets `launches` consecutive launches of one kernel with it.
 static thread_local bool g_capturing = false;  // no timer events inside a capture
 
-KtPair kernel_timer_pair(hipStream_t, int launches) {
+// Library teardown: the kernel timer's event pool.
+__attribute__((destructor)) static void timer_pool_teardown() {
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    for (hipEvent_t e : g_kt_pool) (void)hipEventDestroy(e);
+    g_kt_pool.clear();
+    g_kt_used = 0;
+}
+
+KtPair kernel_timer_pair(hipStream_t, int launches, int kind) {
     KtPair p;
     if (g_capturing) return p;
     std::lock_guard<std::mutex> g(g_kt_mu);
-    if (!g_kt_on) return p;
+    if (!g_kt_on || (g_kt_on == 2 && kind != kTimeSyncFree) || (g_kt_on == 3 && kind != kTimeSweeps)) return p;
     if (g_kt_used + 2 > g_kt_pool.size()) {
         for (int i = 0; i < 64; ++i) {
             hipEvent_t e;
@@ -329,6 +339,7 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     A.trace = (pv->trace && pv->trace_len >= 3 * (size_t)m->nr * k) ? pv->trace : nullptr;
     A.backoff = pv->backoff;
     A.dyn_lds = pv->lds_pad ? pv->lds_pad : lds_pad_for(c.cu_waves, c.wg);
+    A.claim_rounds = c.claim_rounds > 0 ? c.claim_rounds : 1;
     size_t words = (size_t)m->nr * k;
     using word = typename P::word;
     hipLaunchKernelGGL((k_prepare<word>), dim3(stream_grid(words)), dim3(256), 0, s,
@@ -537,9 +548,9 @@ int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, 
     sptrsv_config_t c;
     SPT_TRY(resolve_config(mat, info, precision, k, cfg, &c));
     MatPriv* pv = priv_of(mat);
-    const std::array<long long, 17> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
+    const std::array<long long, 18> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
                                         refine_steps, c.kernel, c.lanes, c.wg, c.fused, c.variant, c.cu_waves,
-                                        c.sweeps, c.outer, c.window, c.sweeps_next, c.inner};
+                                        c.sweeps, c.outer, c.window, c.sweeps_next, c.inner, c.claim_rounds};
     auto it = pv->graphs.find(key);
     if (it == pv->graphs.end()) {
         // one eager solve sets up every workspace and table (no allocation
@@ -705,7 +716,7 @@ int sptrsv_trace_read(sp_mat_t* mat, int64_t* host, size_t count, double* rate_k
 
 int sptrsv_kernel_timer(int enable) {
     std::lock_guard<std::mutex> g(g_kt_mu);
-    g_kt_on = enable != 0;
+    g_kt_on = (enable >= 0 && enable <= 3) ? enable : 1;
     g_kt_used = 0;
     return SPTRSV_OK;
 }
@@ -767,10 +778,12 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
     // several right-hand sides: the persistent walk of the general engine
     // (no ticket per workgroup; the wave count is the residency cap)
     if (K > 1) {
-        for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, 256, 1, 2, 0});
-        if (has_order(info))
-            for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, 256, 1, 2, 0});
-        if (has_waves(info)) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, 256, 1, 2, 0});
+        for (int wg : {256, 1024}) {
+            for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, wg, 1, 2, 0});
+            if (has_order(info))
+                for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, wg, 1, 2, 0});
+            if (has_waves(info)) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1, 2, 0});
+        }
     }
     // lean path (one right-hand side): one row per wave, or single-level waves
     if (K == 1) {
@@ -782,11 +795,14 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
                 for (int wg : {64, 256}) shapes.push_back({SPTRSV_KERNEL_ORDER, L, wg, 1, 1, 0});
         if (has_waves(info))
             for (int wg : {64, 256}) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1, 1, 0});
-        // persistent lean path: a fixed number of resident waves walks the rows
-        for (int L = flo; L <= fhi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, 64, 1, 2, 0});
-        if (has_order(info))
-            for (int L = flo; L <= fhi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, 64, 1, 2, 0});
-        if (has_waves(info)) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, 64, 1, 2, 0});
+        // persistent lean path: a fixed number of resident waves walks the
+        // rows; a workgroup's waves share each claimed block of rows
+        for (int wg : {256, 1024}) {
+            for (int L = flo; L <= fhi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, wg, 1, 2, 0});
+            if (has_order(info))
+                for (int L = flo; L <= fhi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, wg, 1, 2, 0});
+            if (has_waves(info)) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1, 2, 0});
+        }
     }
 
     hipEvent_t e0, e1;
@@ -857,6 +873,20 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
             sptrsv_config_t c = (bc.kernel == kern) ? bc : it->second;
             c.wg = 256;
             c.window = 16;
+            const float t = time_cfg(c);
+            if (status != SPTRSV_OK) break;
+            if (t < best_t) {
+                best_t = t;
+                bc = c;
+            }
+        }
+    }
+    // Stage 4 (persistent walk): other block sizes of the claimed walk.
+    if (status == SPTRSV_OK && bc.variant == 2) {
+        const sptrsv_config_t base = bc;
+        for (int d : {2, 4}) {
+            sptrsv_config_t c = base;
+            c.claim_rounds = d;
             const float t = time_cfg(c);
             if (status != SPTRSV_OK) break;
             if (t < best_t) {

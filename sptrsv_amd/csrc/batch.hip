@@ -22,7 +22,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
+#include <unordered_set>
 #include <vector>
 
 #include "syncfree.hpp"
@@ -127,6 +129,7 @@ struct sptrsv_batch {
     bool serial = false;                 // MIXED: Jacobi half, then the sync-free half, on one stream
     std::vector<long long> ent_off;      // nmat+1: first entry of each matrix
     std::vector<int> max_row;            // per matrix: longest row
+    std::vector<double> chain;           // per matrix: fraction of rows i > 0 that depend on row i - 1
     // MIXED: the sync-free matrices as one matrix of their own, with its own
     // level analysis and tuned configuration; its rows are gathered from and
     // scattered back to the batch order (null: the ticketed task launch)
@@ -148,6 +151,30 @@ struct sptrsv_batch {
 namespace {
 
 constexpr int kBatchWG = 256;
+
+// Batches alive in this process, for the library's teardown.
+std::mutex g_batch_mu;
+std::unordered_set<sptrsv_batch*> g_batches;
+
+// Library teardown, before the HIP runtime's own: captured graphs, streams
+// and events of batches their owners never destroyed (device memory is left
+// to the runtime); no HIP call when there are none.
+__attribute__((destructor)) void batch_teardown() {
+    std::lock_guard<std::mutex> g(g_batch_mu);
+    if (!g_batches.empty()) (void)hipDeviceSynchronize();
+    for (sptrsv_batch* bt : g_batches) {
+        for (auto& gr : bt->graphs) (void)hipGraphExecDestroy(gr.second);
+        bt->graphs.clear();
+        for (hipStream_t* st : {&bt->cap_stream, &bt->aux}) {
+            if (*st) (void)hipStreamDestroy(*st);
+            *st = nullptr;
+        }
+        for (hipEvent_t* ev : {&bt->fork_ev, &bt->join_ev}) {
+            if (*ev) (void)hipEventDestroy(*ev);
+            *ev = nullptr;
+        }
+    }
+}
 
 // Destroys the batch's captured solves: they hold the device addresses of
 // the task tables, the separate matrix and the Jacobi workspaces, so this
@@ -411,8 +438,14 @@ int sptrsv_batch_create(int nmat, const int* n, const int* const* ia, const int*
                         const double* const* a, sptrsv_batch_t** out) {
     if (nmat < 1 || !n || !ia || !ja || !a || !out) return fail(SPTRSV_E_INVALID_ARG, "bad batch arguments");
     auto* bt = new sptrsv_batch();
+    {
+        std::lock_guard<std::mutex> g(g_batch_mu);
+        g_batches.insert(bt);
+    }
     int st = upload_concat(n, ia, ja, a, nmat, &bt->mat, bt->row_off);
     if (st != SPTRSV_OK) {
+        std::lock_guard<std::mutex> g(g_batch_mu);
+        g_batches.erase(bt);
         delete bt;
         return st;
     }
@@ -425,6 +458,7 @@ int sptrsv_batch_create(int nmat, const int* n, const int* const* ia, const int*
     bt->t_sync.assign(nmat, 0.0f);
     bt->ent_off.assign(nmat + 1, 0);
     bt->max_row.assign(nmat, 0);
+    bt->chain.assign(nmat, 0.0);
     long long entry_base = 0;
     for (int m = 0; m < nmat; ++m) {
         long long rows = bt->row_off[m + 1] - bt->row_off[m];
@@ -441,6 +475,12 @@ int sptrsv_batch_create(int nmat, const int* n, const int* const* ia, const int*
             dm = std::min(dm, std::fabs(a[m][ia[m][0] + re - 1]) - off);
         }
         bt->delta[m] = dm;
+        long long pred = 0;  // rows whose largest off-diagonal column is i - 1 (diagonal last)
+        for (int i = 1; i < n[m]; ++i) {
+            const int re = ia[m][i + 1] - ia[m][0];
+            if (re - (ia[m][i] - ia[m][0]) >= 2 && ja[m][ia[m][0] + re - 2] == i - 1) ++pred;
+        }
+        bt->chain[m] = n[m] > 1 ? (double)pred / (n[m] - 1) : 0.0;
         if (rows > 0)
             for (int w = 0; w < 2; ++w)
                 jacobi_host_tiles(ia[m], 0, n[m], bt->row_off[m], entry_base, 1024 << w, bt->mtiles[w][m]);
@@ -692,7 +732,29 @@ int sptrsv_batch_tune(sptrsv_batch_t* bt, int precision, int tune) {
     const int nmat = (int)bt->lanes.size();
     for (int m = 0; m < nmat; ++m) bt->lanes[m] = lanes_rule(bt->avg[m]);
     if (!tune) {
+        // Untuned choice: in MIXED precision a strictly dominant matrix whose
+        // rows mostly depend on their predecessor (a dependency chain about as
+        // long as the matrix, as in banded systems) takes block Jacobi sweeps
+        // with a fixed schedule (5 sweeps x 3 refinement steps, 4 in-tile
+        // iterations, 1024-product tiles); everything else stays sync-free.
+        // The halves run one after the other, the sync-free one as tasks.
         bt->choice.assign(nmat, SPTRSV_KERNEL_SIMPLE);
+        if (precision == SPTRSV_PREC_MIXED)
+            for (int m = 0; m < nmat; ++m)
+                if (bt->delta[m] > 0 && bt->chain[m] >= 0.5) bt->choice[m] = SPTRSV_KERNEL_JACOBI;
+        sptrsv_config_t jc{};
+        jc.kernel = SPTRSV_KERNEL_JACOBI;
+        jc.lanes = 1;
+        jc.wg = 256;
+        jc.fused = 1;
+        jc.variant = 2;
+        jc.sweeps = 5;
+        jc.outer = 3;
+        jc.inner = 4;
+        bt->jac_cfg = jc;
+        bt->serial = true;
+        bt->use_sub = false;
+        free_sub(bt);
         SPT_TRY(install_jacobi(bt));
         return build_tasks(bt);
     }
@@ -755,6 +817,10 @@ int sptrsv_batch_check(sptrsv_batch_t* bt, hipStream_t stream) {
 
 void sptrsv_batch_destroy(sptrsv_batch_t* bt) {
     if (!bt) return;
+    {
+        std::lock_guard<std::mutex> g(g_batch_mu);
+        g_batches.erase(bt);
+    }
     drop_batch_graphs(bt);
     if (bt->d_tasks) (void)hipFree(bt->d_tasks);
     if (bt->d_sub) (void)hipFree(bt->d_sub);

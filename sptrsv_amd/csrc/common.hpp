@@ -102,7 +102,7 @@ struct MatPriv {
     int jac_members = 1;         // independent systems sharing the certificate (batch)
     // captured solves (sptrsv_solve_graph): key = precision, b, x, n, k,
     // refinement steps and the resolved configuration
-    std::map<std::array<long long, 17>, hipGraphExec_t> graphs;
+    std::map<std::array<long long, 18>, hipGraphExec_t> graphs;
     hipStream_t cap_stream = nullptr;
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};
@@ -129,7 +129,9 @@ struct KtPair {
     explicit operator bool() const { return ev[0] != nullptr; }
     hipEvent_t operator[](int i) const { return ev[i]; }
 };
-KtPair kernel_timer_pair(hipStream_t s, int launches = 1);
+constexpr int kTimeSyncFree = 1;   // sync-free solve passes (kernel timer kinds)
+constexpr int kTimeSweeps = 2;     // a refinement step's Jacobi sweeps
+KtPair kernel_timer_pair(hipStream_t s, int launches = 1, int kind = kTimeSyncFree);
 // stable LSD radix sort of (keys, vals) by key in [0, max_key]; tk/tv scratch
 int radix_sort_pairs(int* keys, int* vals, int* tk, int* tv, int n, int max_key, hipStream_t s);
 

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "common.hpp"
@@ -32,6 +33,11 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 }
 
 MatPriv* priv_of(sp_mat_t* m) { return static_cast<MatPriv*>(m->priv); }
+
+// Handles alive in this process, so that the library's teardown can release
+// the HIP objects of handles their owners never freed.
+static std::mutex g_live_mu;
+static std::unordered_set<MatPriv*> g_live;
 
 void drop_graphs(MatPriv* pv) {
     if (!pv || pv->graphs.empty()) return;
@@ -83,6 +89,8 @@ int ensure_priv(sp_mat_t* m) {
     }
     SPT_HIP(hipMemset(pv->counters, 0, 4 * sizeof(int)));
     m->priv = pv;
+    std::lock_guard<std::mutex> g(g_live_mu);
+    g_live.insert(pv);
     return SPTRSV_OK;
 }
 
@@ -109,6 +117,10 @@ void free_mat(sp_mat_t* m) {
         if (pv->jac_norms) (void)hipFree(pv->jac_norms);
         for (auto& g : pv->graphs) (void)hipGraphExecDestroy(g.second);
         if (pv->cap_stream) (void)hipStreamDestroy(pv->cap_stream);
+        {
+            std::lock_guard<std::mutex> g(g_live_mu);
+            g_live.erase(pv);
+        }
         for (TileTable& t : pv->jac_tt) {
             if (t.tiles) (void)hipFree(t.tiles);
             if (t.member) (void)hipFree(t.member);
@@ -613,6 +625,35 @@ int sptrsv_timer_stop(hipStream_t stream, float* ms) {
     return SPTRSV_OK;
 }
 
+static uint4* g_flush_buf = nullptr;  // sptrsv_flush_cache's scratch
+
+}  // extern "C"
+
+namespace sptrsv {
+// Library teardown, before the HIP runtime's own (this library is loaded
+// after it, so it is unloaded first): captured graphs and capture streams of
+// handles still alive, and the library's own events and scratch.  Device
+// memory of live handles is left to the runtime.  Nothing here touches HIP
+// unless such objects exist (a process that never used the GPU stays clear).
+__attribute__((destructor)) static void interface_teardown() {
+    std::lock_guard<std::mutex> g(g_live_mu);
+    if (!g_live.empty() || g_t0 || g_flush_buf) (void)hipDeviceSynchronize();
+    for (MatPriv* pv : g_live) {
+        for (auto& gr : pv->graphs) (void)hipGraphExecDestroy(gr.second);
+        pv->graphs.clear();
+        if (pv->cap_stream) (void)hipStreamDestroy(pv->cap_stream);
+        pv->cap_stream = nullptr;
+    }
+    if (g_t0) (void)hipEventDestroy(g_t0);
+    if (g_t1) (void)hipEventDestroy(g_t1);
+    g_t0 = g_t1 = nullptr;
+    if (g_flush_buf) (void)hipFree(g_flush_buf);
+    g_flush_buf = nullptr;
+}
+}  // namespace sptrsv
+
+extern "C" {
+
 static __global__ void k_flush(uint4* p, size_t count, unsigned salt) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (size_t)gridDim.x * blockDim.x)
@@ -620,11 +661,10 @@ static __global__ void k_flush(uint4* p, size_t count, unsigned salt) {
 }
 
 int sptrsv_flush_cache(hipStream_t stream) {
-    static uint4* buf = nullptr;
     static unsigned salt = 0;
     const size_t bytes = 512ull << 20;  // twice the 256 MB Infinity Cache
-    if (!buf) SPT_HIP(hipMalloc(&buf, bytes));
-    hipLaunchKernelGGL(k_flush, dim3(4096), dim3(256), 0, stream, buf, bytes / sizeof(uint4), ++salt);
+    if (!g_flush_buf) SPT_HIP(hipMalloc(&g_flush_buf, bytes));
+    hipLaunchKernelGGL(k_flush, dim3(4096), dim3(256), 0, stream, g_flush_buf, bytes / sizeof(uint4), ++salt);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }

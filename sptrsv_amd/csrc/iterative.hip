@@ -981,7 +981,7 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
         // the kernel timer brackets the step's consecutive sweeps with one
         // event pair (events between launches would add their own overhead
         // to every sweep)
-        const KtPair ev = kernel_timer_pair(s, sweeps - 1);
+        const KtPair ev = kernel_timer_pair(s, sweeps - 1, kTimeSweeps);
         if (ev) SPT_HIP(hipEventRecord(ev[0], s));
         for (int sw = 1; sw < sweeps; ++sw) {
             const bool last = sw == sweeps - 1;

@@ -32,6 +32,14 @@ std::string lower(const char* s) {
 
 namespace sptrsv {
 int fail(int status, const std::string& msg);
+
+// FNV-1a over a byte range, continuing from h (the checksum of the binary
+// CSR and analysis files).
+unsigned long long fnv1a(const void* p, size_t n, unsigned long long h) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 0x100000001b3ULL;
+    return h;
+}
 }
 
 namespace {
@@ -158,12 +166,7 @@ int read_coo(const char* path, int nthreads, int* m_out, int* nz_out, bool* sym_
     return SPTRSV_OK;
 }
 
-// FNV-1a over a byte range, continuing from h.
-unsigned long long fnv1a(const void* p, size_t n, unsigned long long h) {
-    const unsigned char* c = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 0x100000001b3ULL;
-    return h;
-}
+using sptrsv::fnv1a;
 
 const char kCsrMagic[8] = {'S', 'P', 'T', 'R', 'C', 'S', 'R', '1'};
 }  // namespace

@@ -160,7 +160,10 @@ int csr_L_solve_rocsparse_itsv(sptrsv_rocsparse_t* h, const double* b, double* x
     RS_CHK(a.itsv_solve(h->handle, &nit, free_iter, &tol, nullptr, rocsparse_operation_none, h->mat->nr,
                         h->mat->nnz, &one, h->descr, h->mat->a, h->mat->ia, h->mat->ja, h->info, b, x,
                         rocsparse_solve_policy_auto, h->buffer));
-    if (iters) *iters = nit;
+    // rocSPARSE returns the index k of the stopping test that succeeded (or
+    // the last one); free_iter untested iterations run before each test and
+    // after the last, so (k + 1) * free_iter + k iterations were performed.
+    if (iters) *iters = (nit + 1) * free_iter + nit;
     return SPTRSV_OK;
 }
 

@@ -61,6 +61,7 @@ struct SolveArgs {
     int backoff;            // 0: spin; 1..4: s_sleep between polling turns without progress
     unsigned dyn_lds;       // extra LDS bytes per workgroup (caps resident waves; unused by the kernels)
     long long persist_grid; // persistent kernels: workgroups to launch
+    int claim_rounds;       // persistent kernels: items per wave in each claimed block
 };
 
 template <int K>
@@ -743,85 +744,100 @@ __global__ __launch_bounds__(WG) void k_fast_multirow(SolveArgs A) {
 // ---- persistent lean path --------------------------------------------------------
 // A fixed number of workgroups (the residency cap) walks the work items:
 // rows in natural order (MODE 0), rows in iorder (MODE 1) or multirow waves of
-// the analysis (MODE 2).  Items are claimed, one per wave, from the ticket
-// counter in increasing order, and each wave works its claims in the order it
-// made them; it claims its next item as it starts the current one, so the
-// claim's round trip overlaps the current item's polls.
+// the analysis (MODE 2).  A workgroup claims a block of B = waves * R
+// consecutive items with one atomic on the ticket counter (blocks are handed
+// out in increasing order); its waves take the block's items round-robin,
+// each in increasing order, and a barrier closes the block before the next
+// claim.  One claim per block rather than per item: same-address atomics
+// serialise (about 11 ns each), which one claim per row would make the
+// bound.
 //
-// Deadlock freedom, whatever the number of resident waves (other kernels may
-// hold part of the device): every claimed item is held by a wave that is
-// running.  Let u be the lowest unfinished item; it has been claimed, since a
-// wave only claims after finishing everything it claimed before, and every
-// item it depends on is lower, hence finished.  If u is the current item of
-// its wave, that wave completes it.  If u is its wave's pre-claimed next
-// item, the wave's current item is lower than u, hence finished, and the wave
-// moves on to u.  Either way u completes, so every item does.  A wave that
-// gets an item past the end stops; the kernel ends when every wave has.
-__device__ __forceinline__ int claim_issue(int* ctr, int lane) {
-    return lane == 0 ? atomicAdd(ctr, 1) : 0;
+// Deadlock freedom, whatever the number of resident workgroups (other
+// kernels may hold part of the device): every claimed block belongs to a
+// workgroup that is running, and all of its waves are resident.  Let u be
+// the lowest unfinished item; its block has been claimed (a workgroup claims
+// only after finishing its previous block), and every item it depends on is
+// lower, hence finished.  The wave that owns u in the block has finished its
+// earlier items of the block (they are lower than u), so it is working on u,
+// whose dependencies are all published: u completes.  Waves that are done
+// with their share wait at the barrier holding no item.  So every item
+// completes, and a workgroup leaves when its claim falls past the end.
+template <int WG>
+__device__ __forceinline__ int claim_block(int* ctr, int items_per_block, int* s_slot) {
+    __syncthreads();  // the previous block is done; s_slot may be rewritten
+    if (threadIdx.x == 0) *s_slot = atomicAdd(ctr, 1) * items_per_block;
+    __syncthreads();
+    return *s_slot;
 }
-__device__ __forceinline__ int claim_take(int raw) { return __builtin_amdgcn_readlane(raw, 0); }
 
 template <class P, int WG, int U, int MODE>
 __global__ __launch_bounds__(WG) void k_persist(SolveArgs A) {
+    __shared__ int s_base;
+    constexpr int NW = WG / kWave;
+    const int wv = threadIdx.x >> 6;
     const int lane = threadIdx.x & (kWave - 1);
     const int count = (MODE == 2) ? A.n_waves : A.n;
-    int t = claim_take(claim_issue(A.ticket, lane));
-    while (t < count) {
-        const int next = claim_issue(A.ticket, lane);
-        if constexpr (MODE == 2) {
-            const int vs = A.wsize[t];
-            const int L = vs == 0 ? 1 : vs;
-            const int b0 = A.wbase[t];
-            const int seg = lane / L;
-            const int row = seg < A.wbase[t + 1] - b0 ? A.iorder[b0 + seg] : -1;
-            fast_rows<P, U>(A, L, lane, row);
-        } else {
-            const int row = (MODE == 1) ? A.iorder[t] : t;
-            fast_rows<P, U>(A, A.L, lane, lane < A.L ? row : -1);
+    const int B = NW * max(1, A.claim_rounds);
+    for (int b0 = claim_block<WG>(A.ticket, B, &s_base); b0 < count; b0 = claim_block<WG>(A.ticket, B, &s_base)) {
+        for (int j = wv; j < B && b0 + j < count; j += NW) {
+            const int t = b0 + j;
+            if constexpr (MODE == 2) {
+                const int vs = A.wsize[t];
+                const int L = vs == 0 ? 1 : vs;
+                const int wb = A.wbase[t];
+                const int seg = lane / L;
+                const int row = seg < A.wbase[t + 1] - wb ? A.iorder[wb + seg] : -1;
+                fast_rows<P, U>(A, L, lane, row);
+            } else {
+                const int row = (MODE == 1) ? A.iorder[t] : t;
+                fast_rows<P, U>(A, A.L, lane, lane < A.L ? row : -1);
+            }
         }
-        t = claim_take(next);
     }
 }
 
-// Persistent general engine (any K): the same claimed walk for lane groups
-// of L entry lanes x K right-hand-side lanes.  An item is the 64/(L*K) rows
-// one wave holds, consecutive in natural order (MODE 0) or in iorder (MODE
-// 1), or one multirow wave of the analysis (MODE 2, rows of one level, in
-// passes of 64/(L*K) rows).  Rows of an item may depend on each other in
-// MODE 0/1; sf_rows publishes inside its polling loop, so that is safe, and
-// the deadlock argument is k_persist's.
+// Persistent general engine (any K): the same block-claimed walk for lane
+// groups of L entry lanes x K right-hand-side lanes.  An item is the
+// 64/(L*K) rows one wave holds, consecutive in natural order (MODE 0) or in
+// iorder (MODE 1), or one multirow wave of the analysis (MODE 2, rows of one
+// level, in passes of 64/(L*K) rows).  Rows of an item may depend on each
+// other in MODE 0/1; sf_rows publishes inside its polling loop, so that is
+// safe, and the deadlock argument is k_persist's.
 template <class P, int K, int WG, int U, int MODE>
 __global__ __launch_bounds__(WG) void k_persist_rows(SolveArgs A) {
+    __shared__ int s_base;
+    constexpr int NW = WG / kWave;
+    const int wv = threadIdx.x >> 6;
     const int lane = threadIdx.x & (kWave - 1);
+    const int B = NW * max(1, A.claim_rounds);
     if constexpr (MODE == 2) {
-        int t = claim_take(claim_issue(A.ticket, lane));
-        while (t < A.n_waves) {
-            const int next = claim_issue(A.ticket, lane);
-            const int vs = A.wsize[t];
-            int L = vs == 0 ? 1 : vs;
-            if (L * K > kWave) L = kWave / K;
-            const int b0 = A.wbase[t];
-            const int nrows = A.wbase[t + 1] - b0;
-            const int cap = kWave / (L * K);
-            const int seg = lane / (L * K);
-            for (int c0 = 0; c0 < nrows; c0 += cap) {
-                const int row = (c0 + seg < nrows) ? A.iorder[b0 + c0 + seg] : -1;
-                sf_rows<P, K, U>(A, L, lane, row, nullptr, 0, 0);
+        for (int b0 = claim_block<WG>(A.ticket, B, &s_base); b0 < A.n_waves;
+             b0 = claim_block<WG>(A.ticket, B, &s_base)) {
+            for (int j = wv; j < B && b0 + j < A.n_waves; j += NW) {
+                const int t = b0 + j;
+                const int vs = A.wsize[t];
+                int L = vs == 0 ? 1 : vs;
+                if (L * K > kWave) L = kWave / K;
+                const int wb = A.wbase[t];
+                const int nrows = A.wbase[t + 1] - wb;
+                const int cap = kWave / (L * K);
+                const int seg = lane / (L * K);
+                for (int c0 = 0; c0 < nrows; c0 += cap) {
+                    const int row = (c0 + seg < nrows) ? A.iorder[wb + c0 + seg] : -1;
+                    sf_rows<P, K, U>(A, L, lane, row, nullptr, 0, 0);
+                }
             }
-            t = claim_take(next);
         }
     } else {
         const int lgG = A.lgL + kLog2<K>();
         const int rpw = kWave >> lgG;
         const int items = (A.n + rpw - 1) / rpw;
-        int t = claim_take(claim_issue(A.ticket, lane));
-        while (t < items) {
-            const int next = claim_issue(A.ticket, lane);
-            const int pos = t * rpw + (lane >> lgG);
-            const int row = pos < A.n ? (MODE == 1 ? A.iorder[pos] : pos) : -1;
-            sf_rows<P, K, U>(A, A.L, lane, row, nullptr, 0, 0);
-            t = claim_take(next);
+        for (int b0 = claim_block<WG>(A.ticket, B, &s_base); b0 < items; b0 = claim_block<WG>(A.ticket, B, &s_base)) {
+            for (int j = wv; j < B && b0 + j < items; j += NW) {
+                const int pos = (b0 + j) * rpw + (lane >> lgG);
+                const int row = pos < A.n ? (MODE == 1 ? A.iorder[pos] : pos) : -1;
+                sf_rows<P, K, U>(A, A.L, lane, row, nullptr, 0, 0);
+            }
         }
     }
 }
@@ -932,10 +948,22 @@ hipError_t launch_k(int kind, const SolveArgs& A, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Persistent general engine with 1024-thread workgroups (several right-hand
+// sides): 16 waves share each claimed block, 16 items per claim.
+template <class P, int K, int U>
+hipError_t launch_persist_wide(int kind, const SolveArgs& A, hipStream_t s) {
+    switch (kind) {
+        case KK_PERSIST_SIMPLE: return launch_persist_rows<P, K, 1024, U, 0>(A, s);
+        case KK_PERSIST_ORDER: return launch_persist_rows<P, K, 1024, U, 1>(A, s);
+        case KK_PERSIST_MULTIROW: return launch_persist_rows<P, K, 1024, U, 2>(A, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 // Launch one solve pass.  K: right-hand-side lanes (1..64, power of two);
 // with K == 1 the workgroup may be 64, 256 or 1024 threads and the entry
 // window 1 or 4 (16 with 256 threads); with K > 1 the workgroup is 256
-// threads and the window 1 or 4.
+// threads (1024 for the persistent kinds too) and the window 1 or 4.
 template <class P, int MAXK>
 hipError_t launch_solve(int kind, int K, int wg, int u, const SolveArgs& A, hipStream_t s) {
     if (K == 1) {
@@ -948,6 +976,17 @@ hipError_t launch_solve(int kind, int K, int wg, int u, const SolveArgs& A, hipS
     }
     if constexpr (MAXK > 1) {
         const bool w4 = u >= 4;
+        if (wg == 1024 && kind >= KK_PERSIST_SIMPLE) {
+            switch (K) {
+                case 2: return w4 ? launch_persist_wide<P, 2, 4>(kind, A, s) : launch_persist_wide<P, 2, 1>(kind, A, s);
+                case 4: return w4 ? launch_persist_wide<P, 4, 4>(kind, A, s) : launch_persist_wide<P, 4, 1>(kind, A, s);
+                case 8: return w4 ? launch_persist_wide<P, 8, 4>(kind, A, s) : launch_persist_wide<P, 8, 1>(kind, A, s);
+                case 16: return w4 ? launch_persist_wide<P, 16, 4>(kind, A, s) : launch_persist_wide<P, 16, 1>(kind, A, s);
+                case 32: return w4 ? launch_persist_wide<P, 32, 4>(kind, A, s) : launch_persist_wide<P, 32, 1>(kind, A, s);
+                case 64: return w4 ? launch_persist_wide<P, 64, 4>(kind, A, s) : launch_persist_wide<P, 64, 1>(kind, A, s);
+                default: break;
+            }
+        }
         switch (K) {
             case 2: return w4 ? launch_k<P, 2, 256, 4>(kind, A, s) : launch_k<P, 2, 256, 1>(kind, A, s);
             case 4: return w4 ? launch_k<P, 4, 256, 4>(kind, A, s) : launch_k<P, 4, 256, 1>(kind, A, s);

@@ -173,6 +173,26 @@ def multirow_analysis_base_GPU(mat: SpMat, mode: int = ALL, info: AnalysisInfo |
     return info
 
 
+def save_analysis(path: str, mat: SpMat, info: AnalysisInfo) -> None:
+    """Writes the analysis of mat to a checksummed file (sptrsv_save_analysis)."""
+    check(L.lib().sptrsv_save_analysis(str(path).encode(), byref(mat), byref(info)), "sptrsv_save_analysis")
+
+
+def load_analysis(path: str, mat: SpMat, info: AnalysisInfo | None = None) -> AnalysisInfo:
+    """Reads an analysis file for a matrix of the same structure (sptrsv_load_analysis)."""
+    info = info if info is not None else AnalysisInfo()
+    check(L.lib().sptrsv_load_analysis(str(path).encode(), byref(mat), byref(info)), "sptrsv_load_analysis")
+    return info
+
+
+def copy_analysis(src_mat: SpMat, src: AnalysisInfo, dst_mat: SpMat, dst: AnalysisInfo | None = None) -> AnalysisInfo:
+    """dst_mat (same structure as src_mat) gets its own copy of src (sptrsv_copy_analysis)."""
+    dst = dst if dst is not None else AnalysisInfo()
+    check(L.lib().sptrsv_copy_analysis(byref(src_mat), byref(src), byref(dst_mat), byref(dst)),
+          "sptrsv_copy_analysis")
+    return dst
+
+
 def run_solver(mat: SpMat, info: AnalysisInfo | None, b: np.ndarray) -> np.ndarray:
     """interface.h:8: host b in, host x out."""
     b = np.ascontiguousarray(b, np.float64)
@@ -248,8 +268,9 @@ def csr_L_solve_mixed(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, ref
 
 
 def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0, sweeps=0, outer=0,
-         window=0, sweeps_next=0, inner=0) -> Config:
-    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window, sweeps_next, inner)
+         window=0, sweeps_next=0, inner=0, claim_rounds=0) -> Config:
+    return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window, sweeps_next, inner,
+                  claim_rounds)
 
 
 def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
@@ -533,7 +554,8 @@ class RocsparseBaseline:
 
     def solve_itsv(self, b: DeviceArray, x: DeviceArray, tol: float, max_iter: int = 1000, free_iter: int = 20,
                    stream=None) -> int:
-        """Iterations done (< max_iter when |b - L x|_inf <= tol was reached)."""
+        """Jacobi iterations performed: (k + 1) * free_iter + k, where rocSPARSE's k is the
+        index of the stopping test that met |b - L x|_inf <= tol (max_iter tests at most)."""
         it = c_int()
         check(L.lib().csr_L_solve_rocsparse_itsv(self.h, b.ptr, x.ptr, tol, max_iter, free_iter, byref(it), stream),
               "csr_L_solve_rocsparse_itsv")
@@ -600,8 +622,10 @@ def trace_read(mat: SpMat, k: int = 1) -> tuple[np.ndarray, float]:
     return out.reshape(-1, 3), rate.value
 
 
-def kernel_timer(enable: bool) -> None:
-    check(L.lib().sptrsv_kernel_timer(1 if enable else 0), "kernel_timer")
+def kernel_timer(enable, kind: str = "all") -> None:
+    """HIP events around solve kernels: kind "all", "sync-free" or "sweeps"."""
+    mode = {"all": 1, "sync-free": 2, "sweeps": 3}[kind] if enable else 0
+    check(L.lib().sptrsv_kernel_timer(mode), "kernel_timer")
 
 
 def kernel_timer_read() -> tuple[float, int]:

@@ -453,8 +453,15 @@ def test_rocsparse_baseline(device):
     ri = I.RocsparseBaseline(mat, itsv=True)
     iters = ri.solve_itsv(db, dx, 1e-12, max_iter=2000)
     I.synchronize()
-    assert 0 < iters < 2000
+    assert 20 <= iters < 2000 * 21 and (iters - 20) % 21 == 0  # (k + 1) * 20 + k iterations
     assert relerr(dx.numpy(), ref) <= 1e-10
+    # testing after every iteration finds the first iteration that meets the
+    # bound; testing every 21st finds it at most 20 iterations later
+    every = ri.solve_itsv(db, dx, 1e-12, max_iter=20000, free_iter=0)
+    I.synchronize()
+    assert relerr(dx.numpy(), ref) <= 1e-10
+    print(f"csritsv: {every} iterations when testing each, {iters} when testing every 21st", flush=True)
+    assert every <= iters <= every + 21
     ri.close()
     db.free()
     dx.free()

@@ -86,6 +86,35 @@ def pin_fwd_known_answer():
     return bool(np.all(x32 == 1.0))
 
 
+def fwd_f32_rowloop(ia, ja, a, b):
+    """Serial FP32 forward substitution with every operation rounded to
+    float32 (numpy float32 scalars): piv = 1/a_ii, left -= a_ij * x_j in
+    storage order, x_i = left * piv."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    x = np.zeros(len(ia) - 1, np.float32)
+    one = np.float32(1.0)
+    for i in range(len(ia) - 1):
+        s, e = int(ia[i]), int(ia[i + 1])
+        piv = one / a[e - 1]
+        left = b[i]
+        for p in range(s, e - 1):
+            left = left - a[p] * x[ja[p]]
+        x[i] = left * piv
+    return x
+
+
+def pin_fwd_f32_rowloop():
+    rng = np.random.default_rng(11)
+    for n in (1, 40, 400):
+        ia, ja, a = lower_from_rows(random_rows(n, 7, rng))
+        b = rng.uniform(-1, 1, n)
+        got = O.fwd_f32(ia, ja, a.astype(np.float32), b.astype(np.float32))
+        if not np.array_equal(got.view(np.uint32), fwd_f32_rowloop(ia, ja, a, b).view(np.uint32)):
+            return False
+    return True
+
+
 def pin_levels_closed_form():
     ia, ja, _, (x, y, _) = grid_lower(64, 2, 4.0, -1.0)
     lev, nl = O.levels(ia, ja)
@@ -191,6 +220,7 @@ def pin_loader(golden):
 
 
 PINS = {
+    "fwd_f32_rowloop": lambda g: pin_fwd_f32_rowloop(),
     "class_table": lambda g: pin_class_table(),
     "fwd_scipy": pin_fwd_scipy,
     "fwd_known_answer": lambda g: pin_fwd_known_answer(),
@@ -283,3 +313,11 @@ def test_pins_catch_planted_mistake(m, golden):
     with O.mutation(m):
         failed = [name for name, pin in PINS.items() if not pin(golden)]
     assert failed, f"planted mistake {m} ({MUTATIONS[m]}) passed every pin"
+
+
+@pytest.mark.parametrize("m", [1, 2])
+def test_fwd_f32_pin_catches_planted_mistakes(m):
+    # diagonal from the wrong entry, products added: the float32 row loop
+    # must see either
+    with O.mutation(m):
+        assert not pin_fwd_f32_rowloop()
