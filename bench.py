@@ -96,7 +96,13 @@ KERNEL_NAMES = {1: "simple", 2: "order", 3: "multirow", 4: "jacobi"}
 # Configurations the autotuner chose on MI355X for each workload (sptrsv_config_t
 # fields), used with --preset so that profiler runs time exactly the tuned
 # kernels without running the tuner.
-PRESETS = {}
+PRESETS = {
+    2: {"f64": dict(kernel=2, lanes=16, wg=256, fused=1, variant=2, cu_waves=4),
+        "mixed": dict(kernel=4, lanes=2, wg=256, fused=1, variant=1, sweeps=12, outer=2, window=16)},
+    4: {"f64": dict(kernel=2, lanes=1, wg=1024, fused=1, variant=2, cu_waves=6),
+        "mixed": dict(kernel=2, lanes=1, wg=1024, fused=1, variant=2, cu_waves=12),
+        "f64_per_rhs": dict(kernel=3, lanes=4, wg=256, fused=1, variant=2, cu_waves=6)},
+}
 
 
 def cfg_dict(c) -> dict:
@@ -619,6 +625,9 @@ def main() -> int:
     else:
         b_launch = b_method = bytes_mixed_fused(n, nnz, k)
         kname = "sync-free fused mixed-precision solve"
+        key = (f"fused-mixed[kernel={cm.kernel},lanes={cm.lanes},wg={cm.wg},variant={cm.variant},k={k}]"
+               f"@workload{args.workload}")
+        traffic = profiled_traffic(key)
     achieved = b_launch / (kmsm * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_baseline([s], args.cpu_seconds)
     out = {
