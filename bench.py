@@ -99,6 +99,8 @@ KERNEL_NAMES = {1: "simple", 2: "order", 3: "multirow", 4: "jacobi"}
 PRESETS = {
     2: {"f64": dict(kernel=2, lanes=16, wg=256, fused=1, variant=2, cu_waves=4),
         "mixed": dict(kernel=4, lanes=2, wg=256, fused=1, variant=1, sweeps=12, outer=2, window=16)},
+    3: {"f64": dict(kernel=3, lanes=32, wg=1024, fused=1, variant=2, cu_waves=8),
+        "mixed": dict(kernel=3, lanes=32, wg=1024, fused=1, variant=2, cu_waves=8)},
     4: {"f64": dict(kernel=2, lanes=1, wg=1024, fused=1, variant=2, cu_waves=6),
         "mixed": dict(kernel=2, lanes=1, wg=1024, fused=1, variant=2, cu_waves=12),
         "f64_per_rhs": dict(kernel=3, lanes=4, wg=256, fused=1, variant=2, cu_waves=6)},
@@ -117,6 +119,8 @@ def cfg_dict(c) -> dict:
             c.variant, c.variant), cu_waves=c.cu_waves, window=c.window)
         if c.variant == 2:
             d["claim_rounds"] = c.claim_rounds or 1
+        if c.kernel == 3:
+            d["split_rows"] = c.split_rows >= 0
     return d
 
 

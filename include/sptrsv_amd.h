@@ -214,6 +214,15 @@ int sptrsv_load_analysis(const char* path, sp_mat_t* mat, dfr_analysis_info_t* i
 int sptrsv_copy_analysis(sp_mat_t* src_mat, dfr_analysis_info_t* src, sp_mat_t* dst_mat,
                          dfr_analysis_info_t* dst);
 
+/* Long-row split of a MULTIROW / ALL analysis (no counterpart in the
+ * reference, whose multirow kernel walks every row with at most 32 lanes,
+ * solve_csr_multirow.cu:21-114): a row of more than 256 off-diagonal entries
+ * keeps its first 128 and every further run of 256 entries becomes a segment
+ * that a wave of its own folds; the row adds the segments' partial sums in a
+ * fixed order, so results stay bit-reproducible.  Reports how many rows are
+ * split (may be NULL) and the number of segments. */
+int sptrsv_analysis_segments(const dfr_analysis_info_t* info, int* n_split_rows, int* n_segments);
+
 /* ---- solvers (header/solver.h:5-8) ----------------------------------- */
 /* b and x are device pointers.  x is fully overwritten.  All solves are
  * stream-ordered, allocate nothing after the first call for a given matrix
@@ -266,7 +275,10 @@ int csr_L_solve_multirow_f32(sp_mat_t* mat, dfr_analysis_info_t* info, const flo
  * single-level waves), one right-hand side only; cu_waves caps the waves
  * resident per CU (0 = hardware limit): every resident wave polls, and past a
  * few thousand polling waves the dependency hand-off latency grows several
- * fold, so deep dependency chains run fastest with a small cap. */
+ * fold, so deep dependency chains run fastest with a small cap.  With lanes
+ * = 0 the lanes (and, when variant is 0, the variant) come from the tuned
+ * shape of the kernel, else from the default shape; MULTIROW takes its lanes
+ * per row from the analysis and ignores a nonzero lanes. */
 typedef struct {
     int kernel;
     int lanes;
@@ -288,6 +300,11 @@ typedef struct {
     int claim_rounds; /* persistent variant (2): a workgroup claims blocks of
                          waves * claim_rounds work items with one atomic, a
                          wave taking claim_rounds of them in turn (0 = 1) */
+    int split_rows;   /* MULTIROW with one right-hand side: 0 or 1 splits each
+                         row of more than 256 off-diagonal entries (the row
+                         keeps 128, every further run of 256 is folded by a
+                         wave of its own and added back in a fixed order);
+                         -1 walks whole rows */
 } sptrsv_config_t;
 
 /* Relative accuracy the JACOBI path certifies (default 1e-10). */

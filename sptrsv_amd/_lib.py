@@ -61,7 +61,8 @@ class AnalysisInfo(Structure):  # dfr_analysis_info_t, dfr_syncfree.h:25-41
 class Config(Structure):  # sptrsv_config_t
     _fields_ = [("kernel", c_int), ("lanes", c_int), ("wg", c_int), ("fused", c_int), ("variant", c_int),
                 ("cu_waves", c_int), ("sweeps", c_int), ("outer", c_int), ("window", c_int),
-                ("sweeps_next", c_int), ("inner", c_int), ("claim_rounds", c_int)]
+                ("sweeps_next", c_int), ("inner", c_int), ("claim_rounds", c_int),
+                ("split_rows", c_int)]
 
 
 class BenchResult(Structure):  # sptrsv_bench_t
@@ -97,6 +98,7 @@ _PROTOS = {
     "sptrsv_save_analysis": (c_int, [c_char_p, POINTER(SpMat), POINTER(AnalysisInfo)]),
     "sptrsv_load_analysis": (c_int, [c_char_p, POINTER(SpMat), POINTER(AnalysisInfo)]),
     "sptrsv_copy_analysis": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), POINTER(SpMat), POINTER(AnalysisInfo)]),
+    "sptrsv_analysis_segments": (c_int, [POINTER(AnalysisInfo), POINTER(c_int), POINTER(c_int)]),
     "csr_L_solve_simple": (c_int, [POINTER(SpMat), c_void_p, c_void_p, c_int, c_void_p]),
     "csr_L_solve_order": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int, c_void_p]),
     "csr_L_solve_multirow": (c_int, [POINTER(SpMat), POINTER(AnalysisInfo), c_void_p, c_void_p, c_int, c_void_p]),

@@ -324,14 +324,17 @@ static unsigned grid_for(long long count) {
 // was never filled here (malloc'ed, uninitialised) is recognised by the
 // pointer not being registered, so its garbage pointers are never freed.
 struct InfoPriv {
-    int unused = 0;
+    SegTable seg;              // long-row segments of the multirow walks (derived data)
+    unsigned long long id = 0;  // unique per analysis built, loaded or copied
 };
 static std::mutex g_info_mu;
 static std::unordered_set<void*> g_infos;
+static unsigned long long g_info_ids = 0;
 
 static InfoPriv* new_info_priv() {
     InfoPriv* p = new InfoPriv();
     std::lock_guard<std::mutex> g(g_info_mu);
+    p->id = ++g_info_ids;
     g_infos.insert(p);
     return p;
 }
@@ -341,6 +344,23 @@ static bool owns_info(const dfr_analysis_info_t* info) {
     return info->priv && g_infos.count(info->priv);
 }
 
+unsigned long long analysis_id(const dfr_analysis_info_t* info) {
+    if (!info || !owns_info(info)) return 0;
+    return static_cast<const InfoPriv*>(info->priv)->id;
+}
+
+static void free_segments(SegTable& t) {
+    for (int* p : {t.items, t.seg_lo, t.seg_hi, t.row_seg})
+        if (p) (void)hipFree(p);
+    t = SegTable{};
+}
+
+const SegTable* segments_of(const dfr_analysis_info_t* info) {
+    if (!info || !owns_info(info)) return nullptr;
+    const SegTable& t = static_cast<const InfoPriv*>(info->priv)->seg;
+    return t.items ? &t : nullptr;
+}
+
 void free_info_arrays(dfr_analysis_info_t* info) {
     if (!info || !owns_info(info)) return;
     InfoPriv* ip = static_cast<InfoPriv*>(info->priv);
@@ -348,6 +368,7 @@ void free_info_arrays(dfr_analysis_info_t* info) {
         std::lock_guard<std::mutex> g(g_info_mu);
         g_infos.erase(ip);
     }
+    free_segments(ip->seg);
     int* arrays[] = {info->lev_size, info->warp_lev, info->inv_iorder, info->iorder,
                      info->ibase_row, info->ivect_size_warp, info->row_ctr, info->level,
                      info->bucket_off, info->ibase_row64, info->ivect_size_wave};
@@ -382,6 +403,125 @@ static int build_warps(const int* bucket_cnt, const int* bucket_off, int nb, int
     else
         SPT_HIP(hipFree(wpb));
     return SPTRSV_OK;
+}
+
+// ---- long-row segments -------------------------------------------------------------
+// Rows of the 32-lane class (the longest) with more than kSegMin off-diagonal
+// entries are split: the row keeps its first kSegOwn entries and each further
+// run of kSegEntries becomes a segment item.  The segment items of a wave's
+// rows are placed right before that wave in the walk order.
+
+// Segments each wave needs: the sum of seg_count over its rows.
+__global__ void k_seg_count(const int* __restrict__ ia, const int* __restrict__ iorder,
+                            const int* __restrict__ wbase, const int* __restrict__ wsize, int n_waves, int* cnt) {
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w <= n_waves; w += gridDim.x * blockDim.x) {
+        if (w == n_waves) {
+            cnt[w] = 0;
+            continue;
+        }
+        int c = 0;
+        if (wsize[w] == 32)
+            for (int q = wbase[w]; q < wbase[w + 1]; ++q) {
+                const int r = iorder[q];
+                c += seg_count(ia[r + 1] - ia[r] - 1);
+            }
+        cnt[w] = c;
+    }
+}
+
+// Walk order (item t: wave w >= 0, or segment g as -(g + 1)) and the segment
+// records; seg_off = exclusive scan of k_seg_count, so wave w sits at item
+// w + seg_off[w + 1] with its seg_off[w + 1] - seg_off[w] segments before it.
+__global__ void k_seg_fill(const int* __restrict__ ia, const int* __restrict__ iorder,
+                           const int* __restrict__ wbase, const int* __restrict__ wsize, int n_waves,
+                           const int* __restrict__ seg_off, int* items, int* seg_lo, int* seg_hi, int* row_seg) {
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_waves; w += gridDim.x * blockDim.x) {
+        const int g0 = seg_off[w];
+        const int segs = seg_off[w + 1] - g0;
+        int g = g0;
+        if (segs > 0)
+            for (int q = wbase[w]; q < wbase[w + 1]; ++q) {
+                const int r = iorder[q];
+                const int rs = ia[r], dpos = ia[r + 1] - 1;
+                if (seg_count(dpos - rs) == 0) continue;
+                row_seg[r] = g;
+                for (int lo = rs + kSegOwn; lo < dpos; lo += kSegEntries, ++g) {
+                    items[w + g] = -(g + 1);
+                    seg_lo[g] = lo;
+                    seg_hi[g] = min(lo + kSegEntries, dpos);
+                }
+            }
+        items[w + g0 + segs] = w;
+    }
+}
+
+__global__ void k_count_split(const int* row_seg, int n, int* count) {
+    int c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += row_seg[i] >= 0;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+__global__ void k_fill_i32(int* p, long long n, int v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Builds (or rebuilds) the segment table of an analysis with the wave64
+// layout; nothing when no row needs segments.
+int build_segments(dfr_analysis_info_t* info, sp_mat_t* m) {
+    if (!owns_info(info)) return SPTRSV_OK;
+    SegTable& t = static_cast<InfoPriv*>(info->priv)->seg;
+    free_segments(t);
+    if (!info->ibase_row64 || info->n_waves == 0 || m->nr == 0) return SPTRSV_OK;
+    const int nw = info->n_waves, n = m->nr;
+    hipStream_t s = nullptr;
+    int* cnt = nullptr;
+    SPT_HIP(hipMalloc(&cnt, sizeof(int) * (size_t)(nw + 1)));
+    hipLaunchKernelGGL(k_seg_count, dim3(grid_for(nw + 1)), dim3(256), 0, s, m->ia, info->iorder, info->ibase_row64,
+                       info->ivect_size_wave, nw, cnt);
+    int st = hipGetLastError() == hipSuccess ? SPTRSV_OK : fail(SPTRSV_E_HIP, "segment count");
+    if (st == SPTRSV_OK) st = exclusive_scan(cnt, cnt, nw + 1, s);
+    int total = 0;
+    if (st == SPTRSV_OK && hipMemcpy(&total, cnt + nw, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        st = fail(SPTRSV_E_HIP, "segment total");
+    if (st == SPTRSV_OK && total > 0) {
+        const long long items = (long long)nw + total;
+        if (items > 0x7ffffffeLL) st = fail(SPTRSV_E_CAPACITY, "too many segments");
+        auto alloc = [&](int** p, long long count) {
+            if (st == SPTRSV_OK && hipMalloc(p, sizeof(int) * (size_t)count) != hipSuccess)
+                st = fail(SPTRSV_E_ALLOC, "segment table");
+        };
+        alloc(&t.items, items);
+        alloc(&t.seg_lo, total);
+        alloc(&t.seg_hi, total);
+        alloc(&t.row_seg, n);
+        if (st == SPTRSV_OK) {
+            hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(256), 0, s, t.row_seg, (long long)n, -1);
+            hipLaunchKernelGGL(k_seg_fill, dim3(grid_for(nw)), dim3(256), 0, s, m->ia, info->iorder,
+                               info->ibase_row64, info->ivect_size_wave, nw, cnt, t.items, t.seg_lo, t.seg_hi,
+                               t.row_seg);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+                st = fail(SPTRSV_E_HIP, "segment fill");
+        }
+        t.n_items = (int)items;
+        t.n_segs = total;
+        if (st == SPTRSV_OK) {
+            // rows that own segments
+            int* flag = nullptr;
+            if (hipMalloc(&flag, sizeof(int)) != hipSuccess || hipMemset(flag, 0, sizeof(int)) != hipSuccess)
+                st = fail(SPTRSV_E_ALLOC, "segment count");
+            if (st == SPTRSV_OK) {
+                hipLaunchKernelGGL(k_count_split, dim3(grid_for(n)), dim3(256), 0, s, t.row_seg, n, flag);
+                if (hipMemcpy(&t.n_split, flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+                    st = fail(SPTRSV_E_HIP, "segment count");
+            }
+            if (flag) (void)hipFree(flag);
+        }
+        if (st != SPTRSV_OK) free_segments(t);
+    }
+    (void)hipFree(cnt);
+    return st;
 }
 
 static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
@@ -433,7 +573,8 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
         A.wd_ticks = pv->watchdog_ticks;
         A.dyn_lds = dyn_lds;
         hipLaunchKernelGGL((k_prepare<uint32_t>), dim3(grid_for(n)), dim3(256), 0, s,
-                           reinterpret_cast<uint32_t*>(info->level), (size_t)n, 0u, pv->counters);
+                           reinterpret_cast<uint32_t*>(info->level), (size_t)n, 0u, pv->counters, nullptr,
+                           (size_t)0);
         SPT_HIP(hipGetLastError());
         int u = (pv->max_row > 2 * lanes) ? 4 : 1;
         hipError_t e = launch_solve<PolLevel, 1>(kind, 1, wg, u, A, s);
@@ -524,7 +665,7 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
     (void)hipFree(bucket_cnt);
     if (st != SPTRSV_OK) return st;
     SPT_HIP(hipStreamSynchronize(s));
-    return SPTRSV_OK;
+    return build_segments(info, m);
 }
 
 // ---- analysis files and copies ------------------------------------------------------
@@ -651,6 +792,7 @@ int sptrsv_load_analysis(const char* path, sp_mat_t* mat, dfr_analysis_info_t* i
             hipMemcpy(*a.dev, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
             st = fail(SPTRSV_E_ALLOC, "analysis arrays");
     }
+    if (st == SPTRSV_OK) st = build_segments(info, mat);
     if (st != SPTRSV_OK) free_info_arrays(info);
     return st;
 }
@@ -683,6 +825,7 @@ int sptrsv_copy_analysis(sp_mat_t* src_mat, dfr_analysis_info_t* src, sp_mat_t* 
             hipMemcpy(*to[i].dev, *from[i].dev, bytes, hipMemcpyDeviceToDevice) != hipSuccess)
             st = fail(SPTRSV_E_ALLOC, "analysis arrays");
     }
+    if (st == SPTRSV_OK) st = build_segments(dst, dst_mat);
     if (st != SPTRSV_OK) free_info_arrays(dst);
     return st;
 }
@@ -698,6 +841,14 @@ int multirow_analysis_base_GPU(dfr_analysis_info_t** info, sp_mat_t* gpu_L, MODE
     int st = analyse(*info, gpu_L, mode);
     if (st != SPTRSV_OK) free_info_arrays(*info);
     return st;
+}
+
+int sptrsv_analysis_segments(const dfr_analysis_info_t* info, int* n_split_rows, int* n_segments) {
+    if (!info || !n_segments) return fail(SPTRSV_E_INVALID_ARG, "null info or count");
+    const SegTable* t = segments_of(info);
+    *n_segments = t ? t->n_segs : 0;
+    if (n_split_rows) *n_split_rows = t ? t->n_split : 0;
+    return SPTRSV_OK;
 }
 
 int run_analysis(dfr_analysis_info_t* info, sp_mat_t* gpu_L) {

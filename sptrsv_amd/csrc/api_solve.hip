@@ -102,6 +102,8 @@ static int reference_lanes(const sp_mat_t* m) {
 }
 
 static bool has_order(const dfr_analysis_info_t* info) { return info && info->iorder; }
+// Long rows split into segments unless the configuration asks for whole rows.
+static bool split_rows_on(const sptrsv_config_t& c) { return c.split_rows >= 0; }
 static bool has_waves(const dfr_analysis_info_t* info) {
     return info && info->ibase_row64 && info->ivect_size_wave && info->iorder;
 }
@@ -148,12 +150,13 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     if (cfg && cfg->kernel != SPTRSV_KERNEL_AUTO) {
         *out = *cfg;
         if (cfg->lanes <= 0) {
-            // only the kernel is named: the tuned shape for it, else the default shape
+            // lanes not given: those of the tuned shape for the kernel, else
+            // of the default shape; so is the variant unless one is named
             const sptrsv_config_t& t = pv->tuned_cfg[precision];
             sptrsv_config_t d = (pv->tuned[precision] && pv->tuned_k[precision] == k && t.kernel == cfg->kernel)
                                     ? t : shape_for(m, cfg->kernel, k);
             out->lanes = d.lanes;
-            out->variant = d.variant;
+            if (cfg->variant == 0) out->variant = d.variant;
             if (cfg->wg <= 0) out->wg = d.wg;
         }
     } else if (pv->tuned[precision] && pv->tuned_k[precision] == k &&
@@ -195,6 +198,7 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     }
     if (out->cu_waves < 0) out->cu_waves = 0;
     if (out->claim_rounds < 0 || out->claim_rounds > 64) return fail(SPTRSV_E_INVALID_ARG, "claim_rounds must be 0..64");
+    if (out->split_rows < -1 || out->split_rows > 1) return fail(SPTRSV_E_INVALID_ARG, "split_rows must be -1, 0 or 1");
     if (out->kernel == SPTRSV_KERNEL_JACOBI) {
         if (out->window < 0 || out->window > 64) return fail(SPTRSV_E_INVALID_ARG, "window must be 0..64");
     } else {
@@ -310,11 +314,38 @@ KtPair kernel_timer_pair(hipStream_t, int launches, int kind) {
 
 // ---- one sync-free pass ----------------------------------------------------------------
 
+// Words of segment partials a pass with configuration c needs (0: rows are
+// not split): multirow walks with one right-hand side over an analysis with
+// a segment table.
+static size_t split_part_words(const dfr_analysis_info_t* info, const sptrsv_config_t& c, int k) {
+    if (c.kernel != SPTRSV_KERNEL_MULTIROW || pow2_ceil(k) != 1 || !split_rows_on(c)) return 0;
+    const SegTable* sg = segments_of(info);
+    return sg ? 2 * (size_t)sg->n_segs : 0;
+}
+
+// The matrix's partials workspace sized for `info`'s segment table (before a
+// capture, where no allocation may happen).
+int reserve_part(sp_mat_t* m, const dfr_analysis_info_t* info) {
+    MatPriv* pv = priv_of(m);
+    const SegTable* sg = segments_of(info);
+    const size_t words = sg ? 2 * (size_t)sg->n_segs : 0;
+    if (!pv || pv->part_words >= words) return SPTRSV_OK;
+    drop_graphs(pv);
+    if (pv->part) SPT_HIP(hipFree(pv->part));
+    pv->part = nullptr;
+    pv->part_words = 0;
+    SPT_HIP(hipMalloc(&pv->part, words * sizeof(uint64_t)));
+    pv->part_words = words;
+    return SPTRSV_OK;
+}
+
 // timed = false keeps the pass off the kernel timer (whose event pool may
-// grow, which the allocation-free entry point must not do).
+// grow, which the allocation-free entry point must not do); part_ext, when
+// given, holds the segment partials instead of the matrix's workspace.
 template <class P>
 static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_t& c, int k,
-                    const void* b, void* pub, double* x_out, hipStream_t s, bool timed = true) {
+                    const void* b, void* pub, double* x_out, hipStream_t s, bool timed = true,
+                    uint64_t* part_ext = nullptr) {
     MatPriv* pv = priv_of(m);
     const int K = pow2_ceil(k);
     SolveArgs A{};
@@ -340,11 +371,6 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     A.backoff = pv->backoff;
     A.dyn_lds = pv->lds_pad ? pv->lds_pad : lds_pad_for(c.cu_waves, c.wg);
     A.claim_rounds = c.claim_rounds > 0 ? c.claim_rounds : 1;
-    size_t words = (size_t)m->nr * k;
-    using word = typename P::word;
-    hipLaunchKernelGGL((k_prepare<word>), dim3(stream_grid(words)), dim3(256), 0, s,
-                       static_cast<word*>(pub), words, P::kUnsolved, pv->counters);
-    SPT_HIP(hipGetLastError());
     int kind = c.kernel == SPTRSV_KERNEL_ORDER ? KK_ORDER
                : c.kernel == SPTRSV_KERNEL_MULTIROW ? KK_MULTIROW : KK_SIMPLE;
     if (c.variant == 1 && K == 1) kind += KK_FAST_SIMPLE - KK_SIMPLE;
@@ -355,6 +381,23 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
         const int per_cu = c.cu_waves > 0 ? c.cu_waves : 4;
         A.persist_grid = (long long)device_cus() * std::max(1, per_cu / std::max(1, c.wg / kWave));
     }
+    // long rows split into segments: multirow walks, one right-hand side
+    const size_t part_words = split_part_words(info, c, k);
+    if (part_words) {
+        const SegTable* sg = segments_of(info);
+        if (!part_ext) SPT_TRY(reserve_part(m, info));
+        A.items = sg->items;
+        A.n_items = sg->n_items;
+        A.seg_lo = sg->seg_lo;
+        A.seg_hi = sg->seg_hi;
+        A.row_seg = sg->row_seg;
+        A.part = part_ext ? part_ext : pv->part;
+    }
+    size_t words = (size_t)m->nr * k;
+    using word = typename P::word;
+    hipLaunchKernelGGL((k_prepare<word>), dim3(stream_grid(words)), dim3(256), 0, s,
+                       static_cast<word*>(pub), words, P::kUnsolved, pv->counters, A.part, part_words);
+    SPT_HIP(hipGetLastError());
     int u = c.window > 0 ? c.window : window_for(pv, (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW) ? 32 : c.lanes);
     const KtPair ev = timed ? kernel_timer_pair(s) : KtPair{};
     if (ev) SPT_HIP(hipEventRecord(ev[0], s));
@@ -548,9 +591,11 @@ int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, 
     sptrsv_config_t c;
     SPT_TRY(resolve_config(mat, info, precision, k, cfg, &c));
     MatPriv* pv = priv_of(mat);
-    const std::array<long long, 18> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
+    // the analysis is part of the key: a captured solve holds its arrays
+    const std::array<long long, 20> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
                                         refine_steps, c.kernel, c.lanes, c.wg, c.fused, c.variant, c.cu_waves,
-                                        c.sweeps, c.outer, c.window, c.sweeps_next, c.inner, c.claim_rounds};
+                                        c.sweeps, c.outer, c.window, c.sweeps_next, c.inner, c.claim_rounds,
+                                        c.split_rows, (long long)analysis_id(info)};
     auto it = pv->graphs.find(key);
     if (it == pv->graphs.end()) {
         // one eager solve sets up every workspace and table (no allocation
@@ -642,9 +687,18 @@ int sptrsv_solve_mixed_noalloc(sp_mat_t* mat, dfr_analysis_info_t* info, const s
         return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve runs the sync-free kernels only "
                                           "(JACOBI keeps per-matrix workspaces); name SIMPLE, ORDER or MULTIROW");
     c.fused = 1;
+    // the published words, then (split long rows) the segment partials from
+    // the next 256-byte boundary
+    const size_t pub_words = (size_t)n * k;
+    const size_t part_words = split_part_words(info, c, k);
+    const size_t part_at = (pub_words + 31) / 32 * 32;
+    if (part_words && part_at + part_words > (size_t)SPTRSV_NOALLOC_MAX_WORDS)
+        return fail(SPTRSV_E_CAPACITY, "n * k plus the split rows' " + std::to_string(part_words) +
+                                           " partial words exceed the allocation-free scratch of " +
+                                           std::to_string((long long)SPTRSV_NOALLOC_MAX_WORDS) + " words");
     uint64_t* pub = nullptr;
     SPT_TRY(scratch_on_current_device(&pub));
-    return run_pass<PolMixed>(mat, info, c, k, b, pub, x, stream, false);
+    return run_pass<PolMixed>(mat, info, c, k, b, pub, x, stream, false, part_words ? pub + part_at : nullptr);
 }
 
 int sptrsv_reserve(sp_mat_t* mat, int k) {
@@ -893,6 +947,16 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
                 best_t = t;
                 bc = c;
             }
+        }
+    }
+    // Stage 5 (split long rows): the multirow walk with whole rows instead.
+    if (status == SPTRSV_OK && split_part_words(info, bc, k) > 0) {
+        sptrsv_config_t c = bc;
+        c.split_rows = -1;
+        const float t = time_cfg(c);
+        if (status == SPTRSV_OK && t < best_t) {
+            best_t = t;
+            bc = c;
         }
     }
     (void)hipEventDestroy(e0);

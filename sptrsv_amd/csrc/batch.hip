@@ -39,6 +39,7 @@ int reserve_jacobi(sp_mat_t* m, int k);
 int solve_checked(sp_mat_t* m, dfr_analysis_info_t* info, int precision, const sptrsv_config_t* cfg,
                   const void* b, void* x, int n, int k, int refine_steps, hipStream_t s);
 int convert_values_to_f32(sp_mat_t* m);
+int reserve_part(sp_mat_t* m, const dfr_analysis_info_t* info);
 void jacobi_host_tiles(const int* ia, int r_begin, int r_end, long long row_base, long long entry_base,
                        int window, std::vector<int>& out);
 int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vector<int>* tile_member,
@@ -406,6 +407,7 @@ int build_sub(sptrsv_batch_t* bt) {
     if (st == SPTRSV_OK) st = convert_values_to_f32(&S);
     if (st == SPTRSV_OK) chk(hipDeviceSynchronize(), "sub build");
     if (st == SPTRSV_OK) st = multirow_analysis_base_GPU(&bt->sub_info, &S, ALL);
+    if (st == SPTRSV_OK) st = reserve_part(&S, bt->sub_info);  // captured solves allocate nothing
     if (st != SPTRSV_OK) free_sub(bt);
     return st;
 }

@@ -61,6 +61,8 @@ struct MatPriv {
     int* counters = nullptr;     // [0] ticket, [1] watchdog word, [2..] spare (device)
     uint64_t* pub = nullptr;     // published words, n*k 64-bit (device)
     size_t pub_words = 0;
+    uint64_t* part = nullptr;    // published partial sums of long-row segments, 2 per segment (device)
+    size_t part_words = 0;
     double* resid = nullptr;     // two-pass refinement buffers (device)
     float* corr = nullptr;
     float* b32 = nullptr;
@@ -102,12 +104,46 @@ struct MatPriv {
     int jac_members = 1;         // independent systems sharing the certificate (batch)
     // captured solves (sptrsv_solve_graph): key = precision, b, x, n, k,
     // refinement steps and the resolved configuration
-    std::map<std::array<long long, 18>, hipGraphExec_t> graphs;
+    std::map<std::array<long long, 20>, hipGraphExec_t> graphs;
     hipStream_t cap_stream = nullptr;
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};
     int tuned_k[3] = {0, 0, 0};
 };
+
+// Long-row segments of an analysis (its wave64 multirow layout).  A row of
+// more than kSegMin off-diagonal entries keeps its first kSegOwn entries;
+// the rest is cut into runs of kSegEntries, each a segment item: one wave
+// folds the run in a fixed order and publishes the partial sum, and the row
+// adds its partials in segment order.  The multirow walks go through
+// items[] (wave w >= 0, segment g as -(g + 1)); the segments of a wave's rows
+// come right before that wave, so an item only ever waits on lower items.
+// The arrays live on the device.
+constexpr int kSegMin = 256;
+constexpr int kSegOwn = 128;
+constexpr int kSegEntries = 256;
+// Segments of a row with `off` off-diagonal entries.
+__host__ __device__ constexpr int seg_count(int off) {
+    return off > kSegMin ? (off - kSegOwn + kSegEntries - 1) / kSegEntries : 0;
+}
+// Published partial words: all-ones = not yet published; the poison pattern
+// (NaN in every policy's view) stands in for a partial the watchdog gave up on.
+constexpr uint64_t kPartPoison = 0x7ff800007fc00000ull;
+struct SegTable {
+    int n_items = 0;         // n_waves + n_segs
+    int n_segs = 0;
+    int n_split = 0;         // rows that own segments
+    int* items = nullptr;    // [n_items]
+    int* seg_lo = nullptr;   // first entry of each segment [n_segs]
+    int* seg_hi = nullptr;   // end entry of each segment [n_segs]
+    int* row_seg = nullptr;  // first segment of each row, -1 for none [n]
+};
+// The analysis's segment table, or null when it has none.
+const SegTable* segments_of(const dfr_analysis_info_t* info);
+int build_segments(dfr_analysis_info_t* info, sp_mat_t* m);
+// Identifies an analysis for the captured-solve cache: unique per analysis
+// built, loaded or copied (never reused); 0 for none.
+unsigned long long analysis_id(const dfr_analysis_info_t* info);
 
 MatPriv* priv_of(sp_mat_t* m);
 int ensure_priv(sp_mat_t* m);

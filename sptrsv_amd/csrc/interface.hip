@@ -104,6 +104,7 @@ void free_mat(sp_mat_t* m) {
     if (pv) {
         if (pv->counters) (void)hipFree(pv->counters);
         if (pv->pub) (void)hipFree(pv->pub);
+        if (pv->part) (void)hipFree(pv->part);
         if (pv->b32) (void)hipFree(pv->b32);
         if (pv->corr) (void)hipFree(pv->corr);
         if (pv->resid) (void)hipFree(pv->resid);

@@ -62,6 +62,14 @@ struct SolveArgs {
     unsigned dyn_lds;       // extra LDS bytes per workgroup (caps resident waves; unused by the kernels)
     long long persist_grid; // persistent kernels: workgroups to launch
     int claim_rounds;       // persistent kernels: items per wave in each claimed block
+    // long-row segments of the multirow walks, one right-hand side (null
+    // items: rows are never split)
+    const int* items;       // walk order: wave w >= 0, segment g as -(g + 1)
+    int n_items;
+    const int* seg_lo;      // entries [seg_lo, seg_hi) of segment g
+    const int* seg_hi;
+    const int* row_seg;     // first segment of each row, -1 for none
+    uint64_t* part;         // partial sums, 2 words per segment (all-ones = not published)
 };
 
 template <int K>
@@ -188,6 +196,10 @@ struct PolF64 {
     }
     __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
     __device__ static Out bcast_out(const Out& o, int) { return o; }
+    // a segment's partial sum as published words (never all-ones)
+    static constexpr int kPartWords = 1;
+    __device__ static void part_pack(const Acc& a, uint64_t* w) { w[0] = canon64(a.s); }
+    __device__ static Acc part_unpack(const uint64_t* w) { return {__longlong_as_double((long long)w[0])}; }
 };
 
 // Plain FP32 on the FP32 copy of the values, published into x (float).
@@ -216,6 +228,9 @@ struct PolF32 {
     }
     __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
     __device__ static Out bcast_out(const Out& o, int) { return o; }
+    static constexpr int kPartWords = 1;
+    __device__ static void part_pack(const Acc& a, uint64_t* w) { w[0] = canon32(a.s); }
+    __device__ static Acc part_unpack(const uint64_t* w) { return {__uint_as_float((uint32_t)w[0])}; }
 };
 
 // Fused mixed precision, one refinement step in the same dependency pass:
@@ -272,6 +287,15 @@ struct PolMixed {
         uint32_t hi = __float_as_uint(d);
         return ((uint64_t)hi << 32) | lo;  // never all-ones: lo is canonical
     }
+    static constexpr int kPartWords = 2;
+    __device__ static void part_pack(const Acc& a, uint64_t* w) {
+        w[0] = ((uint64_t)canon32(a.d) << 32) | canon32(a.x0);
+        w[1] = canon64(a.r);
+    }
+    __device__ static Acc part_unpack(const uint64_t* w) {
+        return {__uint_as_float((uint32_t)w[0]), __uint_as_float((uint32_t)(w[0] >> 32)),
+                __longlong_as_double((long long)w[1])};
+    }
 };
 
 // Level analysis: lev_i = 1 + max lev_j (analysis_csr.cu:147-181).  0 marks
@@ -297,6 +321,9 @@ struct PolLevel {
     }
     __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
     __device__ static Out bcast_out(const Out& o, int) { return o; }
+    static constexpr int kPartWords = 1;
+    __device__ static void part_pack(const Acc& a, uint64_t* w) { w[0] = (uint64_t)(uint32_t)a.m; }
+    __device__ static Acc part_unpack(const uint64_t* w) { return {(int)(uint32_t)w[0]}; }
 };
 
 // All lanes of a group of G lanes (K right-hand-side lanes interleaved)
@@ -318,7 +345,7 @@ __device__ __forceinline__ typename P::Acc group_reduce(typename P::Acc t, int G
 // lanes (power of two, lanes with r >= A.k idle), L*K divides 64.
 // s_pub/lds_lo/lds_hi: rows in [lds_lo, lds_hi) are also published to and
 // read from LDS (natural order only; pass lds_hi = lds_lo to disable).
-template <class P, int K, int U>
+template <class P, int K, int U, bool SPLIT = false>
 __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int row,
                                         typename P::word* s_pub, int lds_lo, int lds_hi) {
     using word = typename P::word;
@@ -337,6 +364,21 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
         rs = A.ia[row];
         dpos = A.ia[row + 1] - 1;
     }
+    // A split long row (SPLIT: multirow walks with a segment table, one
+    // right-hand side) folds its first kSegOwn entries, then its segments'
+    // partial sums: lane e adds partials e, e + L, ... in that order.
+    int eend = dpos;
+    int pg = e, pg_end = 0;  // this lane's next partial (absolute index), end of the row's partials
+    if constexpr (SPLIT && K == 1) {
+        if (A.items && live && dpos - rs > kSegMin) {
+            const int g = A.row_seg[row];
+            if (g >= 0) {
+                pg = g + e;
+                pg_end = g + seg_count(dpos - rs);
+                eend = rs + kSegOwn;
+            }
+        }
+    }
     const size_t xi = (size_t)(live ? row : 0) * A.k + r;
     Acc acc = (live && e == 0) ? P::seed(A, xi) : P::zero();
     // The diagonal (and its reciprocal) is independent of the dependencies:
@@ -346,7 +388,8 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
 
     int p = rs + e;     // first entry of the current window
     int kc = 0;         // entries of the window already folded
-    bool done = !(live && p < dpos);
+    bool edone = !(live && p < eend);  // entries folded
+    bool done = edone && pg >= pg_end;  // entries and partials folded
     bool published = !live;
 
     int col[U];
@@ -355,7 +398,7 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
     for (int u = 0; u < U; ++u) {
         int q = p + u * L;
         col[u] = 0;
-        if (!done && q < dpos) {
+        if (!edone && q < eend) {
             col[u] = A.ja[q];
             val[u] = P::load(A, q);
         }
@@ -373,7 +416,7 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
         for (int u = 0; u < U; ++u) {
             const int q = p + U * L + u * L;
             ncol[u] = 0;
-            if (!done && q < dpos) {
+            if (!edone && q < eend) {
                 ncol[u] = A.ja[q];
                 nval[u] = P::load(A, q);
             }
@@ -405,7 +448,7 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
             const int q = p + u * L;
             w[u] = P::kUnsolved;
             in_lds[u] = false;
-            if (!done && u >= kc && q < dpos) {
+            if (!edone && u >= kc && q < eend) {
                 const int c = col[u];
                 if (c >= lds_lo && c < lds_hi)
                     in_lds[u] = true;
@@ -413,6 +456,12 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
                     w[u] = poll_global(&gpub[(size_t)c * A.k + r]);
             }
         }
+        // the next partial of a split row whose entries are folded
+        uint64_t pw[P::kPartWords];
+        const bool ppoll = SPLIT && (K == 1) && edone && !done;
+#pragma unroll
+        for (int i = 0; i < P::kPartWords; ++i)
+            pw[i] = ppoll ? poll_global(&A.part[2 * (size_t)pg + i]) : kUnsolved64;
         asm volatile("" ::: "memory");
         // (3) LDS polls, after this wave's own LDS stores; (4) fold the ready
         // prefix of the window in storage order.  LDS results live in their
@@ -422,20 +471,37 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 wl[u] = in_lds[u] ? poll_lds(&s_pub[col[u] - lds_lo]) : P::kUnsolved;
-            if (!done) {
+            if (ppoll) {
+                bool ok = true;
+#pragma unroll
+                for (int i = 0; i < P::kPartWords; ++i) ok &= pw[i] != kUnsolved64;
+                if (!ok && expired) {
+#pragma unroll
+                    for (int i = 0; i < P::kPartWords; ++i) pw[i] = kPartPoison;
+                    ok = true;
+                }
+                if (ok) {
+                    acc = P::add(acc, P::part_unpack(pw));
+                    moved = true;
+                    pg += L;
+                    done = pg >= pg_end;
+                }
+            }
+            if (!edone) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int q = p + u * L;
                     word v = in_lds[u] ? wl[u] : w[u];
-                    if (expired && u >= kc && q < dpos && v == P::kUnsolved) v = P::kPoison;
-                    if (u == kc && q < dpos && v != P::kUnsolved) {
+                    if (expired && u >= kc && q < eend && v == P::kUnsolved) v = P::kPoison;
+                    if (u == kc && q < eend && v != P::kUnsolved) {
                         P::fold(acc, val[u], v);
                         moved = true;
                         ++kc;
                     }
                 }
-                if (p + kc * L >= dpos) {
-                    done = true;
+                if (p + kc * L >= eend) {
+                    edone = true;
+                    done = pg >= pg_end;
                 } else if (kc == U) {
                     p += U * L;
                     kc = 0;
@@ -445,7 +511,7 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
                             col[u] = ncol[u];
                             val[u] = nval[u];
                             const int q = p + U * L + u * L;
-                            if (q < dpos) {
+                            if (q < eend) {
                                 ncol[u] = A.ja[q];
                                 nval[u] = P::load(A, q);
                             }
@@ -454,7 +520,7 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
                             const int q = p + u * L;
-                            if (q < dpos) {
+                            if (q < eend) {
                                 col[u] = A.ja[q];
                                 val[u] = P::load(A, q);
                             }
@@ -559,18 +625,29 @@ __global__ __launch_bounds__(WG) void k_order(SolveArgs A) {
     sf_rows<P, K, U>(A, A.L, lane, row, nullptr, 0, 0);
 }
 
+// Multirow work item t: with a segment table, items[t] (a wave, or segment g
+// as -(g + 1)); else wave t.
+__device__ __forceinline__ int multirow_item(const SolveArgs& A, int t) { return A.items ? A.items[t] : t; }
+__device__ __forceinline__ int multirow_items(const SolveArgs& A) { return A.items ? A.n_items : A.n_waves; }
+
 // (level, class) packed waves (multirow): wave w solves the rows at iorder
 // positions [wbase[w], wbase[w+1]) with wsize[w] lanes per row.  Rows of one
-// wave share a level, so they never depend on each other.
+// wave share a level, so they never depend on each other.  Segment items
+// (one right-hand side) are folded by the whole wave.
 template <class P, int K, int WG, int U>
 __global__ __launch_bounds__(WG) void k_multirow(SolveArgs A) {
     __shared__ int s_t;
     const int tid = threadIdx.x;
     if (tid == 0) s_t = atomicAdd(A.ticket, 1);
     __syncthreads();
-    const int w = s_t * (WG / kWave) + (tid >> 6);
-    if (w >= A.n_waves) return;
+    const int t = s_t * (WG / kWave) + (tid >> 6);
+    if (t >= multirow_items(A)) return;
+    const int w = multirow_item(A, t);
     const int lane = tid & (kWave - 1);
+    if (w < 0) {
+        if constexpr (K == 1) fast_segment<P, U>(A, lane, -w - 1);
+        return;
+    }
     const int vs = A.wsize[w];
     int L = vs == 0 ? 1 : vs;
     if (L * K > kWave) L = kWave / K;
@@ -582,7 +659,7 @@ __global__ __launch_bounds__(WG) void k_multirow(SolveArgs A) {
     // rows than lane groups; its rows share a level, so passes are independent.
     for (int c0 = 0; c0 < nrows; c0 += cap) {
         const int row = (c0 + seg < nrows) ? A.iorder[b0 + c0 + seg] : -1;
-        sf_rows<P, K, U>(A, L, lane, row, nullptr, 0, 0);
+        sf_rows<P, K, U, true>(A, L, lane, row, nullptr, 0, 0);
     }
 }
 
@@ -595,27 +672,15 @@ __global__ __launch_bounds__(WG) void k_multirow(SolveArgs A) {
 // which keeps the detection delay close to the memory round trip.  Chunks are
 // U entries per lane (for U <= 4 the next chunk's entries load while the
 // current one is polled); folding order per lane is the same as sf_rows'.
+// Folds entries [lo, hi) into acc: lane e of a group of L lanes takes
+// entries lo + e, lo + e + L, ... in that order, in chunks of U whose
+// dependencies are all present before the chunk is folded.
 template <class P, int U>
-__device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, int row) {
+__device__ __forceinline__ void fold_entries(const SolveArgs& A, int L, int e, int lane, bool live, int lo, int hi,
+                                             typename P::Acc& acc, long long t0, bool& expired, int& spins) {
     using word = typename P::word;
-    using Acc = typename P::Acc;
     using Val = typename P::Val;
-    word* gpub = static_cast<word*>(A.pub);
-    const int e = lane & (L - 1);
-    const bool live = row >= 0;
-    int rs = 0, dpos = 0;
-    if (live) {
-        rs = A.ia[row];
-        dpos = A.ia[row + 1] - 1;
-    }
-    const size_t xi = live ? (size_t)row : 0;
-    Acc acc = (live && e == 0) ? P::seed(A, xi) : P::zero();
-    typename P::Diag dg{};
-    if (live && e == 0) dg = P::diag(A, dpos);
-    if (A.trace && live && e == 0) A.trace[3 * xi] = wall_clock64();
-    const long long t0 = wall_clock64();
-    bool expired = false;
-    int spins = 0;
+    const word* gpub = static_cast<const word*>(A.pub);
     // Entries of the next chunk are loaded before the current chunk is
     // polled (chunks of up to 4 entries per lane), so a long row pays the
     // entry-load latency once, not per chunk.
@@ -624,14 +689,14 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
     Val val[U], nval[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int q = rs + e + u * L;
+        const int q = lo + e + u * L;
         col[u] = 0;
-        if (live && q < dpos) {
+        if (live && q < hi) {
             col[u] = A.ja[q];
             val[u] = P::load(A, q);
         }
     }
-    for (int p = rs + e; __ballot(live && p < dpos); p += U * L) {
+    for (int p = lo + e; __ballot(live && p < hi); p += U * L) {
         word got[U];
         bool pend[U];
         if constexpr (kPrefetch) {
@@ -639,17 +704,17 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
             for (int u = 0; u < U; ++u) {
                 const int q = p + U * L + u * L;
                 ncol[u] = 0;
-                if (live && q < dpos) {
+                if (live && q < hi) {
                     ncol[u] = A.ja[q];
                     nval[u] = P::load(A, q);
                 }
             }
-        } else if (p != rs + e) {
+        } else if (p != lo + e) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int q = p + u * L;
                 col[u] = 0;
-                if (live && q < dpos) {
+                if (live && q < hi) {
                     col[u] = A.ja[q];
                     val[u] = P::load(A, q);
                 }
@@ -657,7 +722,7 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            pend[u] = live && p + u * L < dpos;
+            pend[u] = live && p + u * L < hi;
             got[u] = P::kUnsolved;
         }
         while (true) {
@@ -688,7 +753,7 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (live && p + u * L < dpos) P::fold(acc, val[u], got[u]);
+            if (live && p + u * L < hi) P::fold(acc, val[u], got[u]);
         if constexpr (kPrefetch) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -696,6 +761,72 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
                 val[u] = nval[u];
             }
         }
+    }
+}
+
+template <class P, int U, bool SPLIT = false>
+__device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, int row) {
+    using word = typename P::word;
+    using Acc = typename P::Acc;
+    word* gpub = static_cast<word*>(A.pub);
+    const int e = lane & (L - 1);
+    const bool live = row >= 0;
+    int rs = 0, dpos = 0;
+    if (live) {
+        rs = A.ia[row];
+        dpos = A.ia[row + 1] - 1;
+    }
+    // a split long row (SPLIT: multirow walks with a segment table) folds its
+    // first kSegOwn entries, then its segments' partial sums (lane e:
+    // partials e, e + L, ...)
+    int eend = dpos, seg0 = 0, nseg = 0;
+    if constexpr (SPLIT) {
+        if (A.items && live && dpos - rs > kSegMin) {
+            const int g = A.row_seg[row];
+            if (g >= 0) {
+                seg0 = g;
+                nseg = seg_count(dpos - rs);
+                eend = rs + kSegOwn;
+            }
+        }
+    }
+    const size_t xi = live ? (size_t)row : 0;
+    Acc acc = (live && e == 0) ? P::seed(A, xi) : P::zero();
+    typename P::Diag dg{};
+    if (live && e == 0) dg = P::diag(A, dpos);
+    if (A.trace && live && e == 0) A.trace[3 * xi] = wall_clock64();
+    const long long t0 = wall_clock64();
+    bool expired = false;
+    int spins = 0;
+    fold_entries<P, U>(A, L, e, lane, live, rs, eend, acc, t0, expired, spins);
+    for (int j = e; SPLIT && __ballot(j < nseg); j += L) {
+        uint64_t pw[P::kPartWords];
+        bool pend = j < nseg;
+        while (true) {
+            if (pend) {
+                bool ok = true;
+#pragma unroll
+                for (int i = 0; i < P::kPartWords; ++i) {
+                    pw[i] = poll_global(&A.part[2 * (size_t)(seg0 + j) + i]);
+                    ok &= pw[i] != kUnsolved64;
+                }
+                if (!ok && expired) {
+#pragma unroll
+                    for (int i = 0; i < P::kPartWords; ++i) pw[i] = kPartPoison;
+                    ok = true;
+                }
+                pend = !ok;
+            }
+            if (!__ballot(pend)) break;
+            if (++spins == 512) {
+                spins = 0;
+                if (!expired && wall_clock64() - t0 > A.wd_ticks) {
+                    expired = true;
+                    if (lane == 0) atomicOr(A.err, 1);
+                }
+            }
+        }
+        if (j < nseg) acc = P::add(acc, P::part_unpack(pw));
     }
     const Acc t = group_reduce<P, 1>(acc, L);
     if (live && e == 0) {
@@ -705,6 +836,25 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
         P::emit(A, xi, o);
         if (A.trace) A.trace[3 * xi + 2] = wall_clock64();
         publish_global(&gpub[xi], w);
+    }
+}
+
+// Segment item g of a split long row: the whole wave folds entries
+// [seg_lo[g], seg_hi[g]) from zero and publishes the partial sum.
+template <class P, int U>
+__device__ __forceinline__ void fast_segment(const SolveArgs& A, int lane, int g) {
+    using Acc = typename P::Acc;
+    const long long t0 = wall_clock64();
+    bool expired = false;
+    int spins = 0;
+    Acc acc = P::zero();
+    fold_entries<P, U>(A, kWave, lane, lane, true, A.seg_lo[g], A.seg_hi[g], acc, t0, expired, spins);
+    const Acc t = group_reduce<P, 1>(acc, kWave);
+    if (lane == 0) {
+        uint64_t w[P::kPartWords];
+        P::part_pack(t, w);
+#pragma unroll
+        for (int i = 0; i < P::kPartWords; ++i) publish_global(&A.part[2 * (size_t)g + i], w[i]);
     }
 }
 
@@ -729,16 +879,21 @@ __global__ __launch_bounds__(WG) void k_fast_multirow(SolveArgs A) {
     const int tid = threadIdx.x;
     if (tid == 0) s_t = atomicAdd(A.ticket, 1);
     __syncthreads();
-    const int w = s_t * (WG / kWave) + (tid >> 6);
-    if (w >= A.n_waves) return;
+    const int t = s_t * (WG / kWave) + (tid >> 6);
+    if (t >= multirow_items(A)) return;
+    const int w = multirow_item(A, t);
     const int lane = tid & (kWave - 1);
+    if (w < 0) {
+        fast_segment<P, U>(A, lane, -w - 1);
+        return;
+    }
     const int vs = A.wsize[w];
     const int L = vs == 0 ? 1 : vs;
     const int b0 = A.wbase[w];
     const int nrows = A.wbase[w + 1] - b0;
     const int seg = lane / L;
     const int row = seg < nrows ? A.iorder[b0 + seg] : -1;
-    fast_rows<P, U>(A, L, lane, row);
+    fast_rows<P, U, true>(A, L, lane, row);
 }
 
 // ---- persistent lean path --------------------------------------------------------
@@ -776,18 +931,23 @@ __global__ __launch_bounds__(WG) void k_persist(SolveArgs A) {
     constexpr int NW = WG / kWave;
     const int wv = threadIdx.x >> 6;
     const int lane = threadIdx.x & (kWave - 1);
-    const int count = (MODE == 2) ? A.n_waves : A.n;
+    const int count = (MODE == 2) ? multirow_items(A) : A.n;
     const int B = NW * max(1, A.claim_rounds);
     for (int b0 = claim_block<WG>(A.ticket, B, &s_base); b0 < count; b0 = claim_block<WG>(A.ticket, B, &s_base)) {
         for (int j = wv; j < B && b0 + j < count; j += NW) {
             const int t = b0 + j;
             if constexpr (MODE == 2) {
-                const int vs = A.wsize[t];
+                const int w = multirow_item(A, t);
+                if (w < 0) {
+                    fast_segment<P, U>(A, lane, -w - 1);
+                    continue;
+                }
+                const int vs = A.wsize[w];
                 const int L = vs == 0 ? 1 : vs;
-                const int wb = A.wbase[t];
+                const int wb = A.wbase[w];
                 const int seg = lane / L;
-                const int row = seg < A.wbase[t + 1] - wb ? A.iorder[wb + seg] : -1;
-                fast_rows<P, U>(A, L, lane, row);
+                const int row = seg < A.wbase[w + 1] - wb ? A.iorder[wb + seg] : -1;
+                fast_rows<P, U, true>(A, L, lane, row);
             } else {
                 const int row = (MODE == 1) ? A.iorder[t] : t;
                 fast_rows<P, U>(A, A.L, lane, lane < A.L ? row : -1);
@@ -842,12 +1002,15 @@ __global__ __launch_bounds__(WG) void k_persist_rows(SolveArgs A) {
     }
 }
 
-// Fill the published words with the unsolved pattern and reset the ticket.
+// Fill the published words with the unsolved pattern, the segment partials
+// (if any) with all-ones, and reset the ticket.
 template <class T>
-__global__ void k_prepare(T* pub, size_t count, T unsolved, int* ticket) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *ticket = 0;
-    for (; i < count; i += (size_t)gridDim.x * blockDim.x) pub[i] = unsolved;
+__global__ void k_prepare(T* pub, size_t count, T unsolved, int* ticket, uint64_t* part, size_t part_count) {
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    if (i0 == 0) *ticket = 0;
+    for (size_t i = i0; i < count; i += stride) pub[i] = unsolved;
+    for (size_t i = i0; i < part_count; i += stride) part[i] = kUnsolved64;
 }
 
 // ---- host-side launch ----------------------------------------------------------
@@ -894,7 +1057,7 @@ template <class P, int K, int WG, int U>
 hipError_t launch_k(int kind, const SolveArgs& A, hipStream_t s) {
     unsigned grid;
     if (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW)
-        grid = ceil_div(A.n_waves, WG / kWave);
+        grid = ceil_div(A.items ? A.n_items : A.n_waves, WG / kWave);
     else if (kind == KK_FAST_SIMPLE || kind == KK_FAST_ORDER)
         grid = ceil_div(A.n, WG / kWave);
     else

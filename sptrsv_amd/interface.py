@@ -185,6 +185,13 @@ def load_analysis(path: str, mat: SpMat, info: AnalysisInfo | None = None) -> An
     return info
 
 
+def analysis_segments(info: AnalysisInfo) -> tuple[int, int]:
+    """(rows split, segments) of a MULTIROW / ALL analysis (sptrsv_analysis_segments)."""
+    rows, segs = c_int(), c_int()
+    check(L.lib().sptrsv_analysis_segments(byref(info), byref(rows), byref(segs)), "sptrsv_analysis_segments")
+    return rows.value, segs.value
+
+
 def copy_analysis(src_mat: SpMat, src: AnalysisInfo, dst_mat: SpMat, dst: AnalysisInfo | None = None) -> AnalysisInfo:
     """dst_mat (same structure as src_mat) gets its own copy of src (sptrsv_copy_analysis)."""
     dst = dst if dst is not None else AnalysisInfo()
@@ -268,9 +275,9 @@ def csr_L_solve_mixed(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, ref
 
 
 def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0, sweeps=0, outer=0,
-         window=0, sweeps_next=0, inner=0, claim_rounds=0) -> Config:
+         window=0, sweeps_next=0, inner=0, claim_rounds=0, split_rows=0) -> Config:
     return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window, sweeps_next, inner,
-                  claim_rounds)
+                  claim_rounds, split_rows)
 
 
 def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
