@@ -225,6 +225,19 @@ def cpu_baseline(systems, budget_s: float):
                       f"{_cpu_model()}, 1 of {os.cpu_count()} cores used"}
 
 
+def calibrate_copy(I) -> None:
+    """Five copies of 256 MiB with the library's k_copy16: the known-size
+    launches the FETCH_SIZE / WRITE_SIZE counters are scaled against."""
+    src = I.DeviceArray(32 << 20, np.float64)                       # 256 MiB
+    dst = I.DeviceArray(32 << 20, np.float64)
+    for _ in range(5):
+        I.copy_device(dst, src, 256 << 20)
+    I.synchronize()
+    src.free()
+    dst.free()
+    log("[bench] calibration: 5 k_copy16 copies of 262144 KiB (read 262144 KiB, write 262144 KiB each)")
+
+
 def run_batch(args, I, O) -> int:
     """Workload 5: 256 independent systems in one captured graph (MIXED, per-matrix
     kernel choice) against the same systems solved one after another in plain FP64."""
@@ -276,42 +289,49 @@ def run_batch(args, I, O) -> int:
         xm = x[bt.row_off[m]:bt.row_off[m + 1]]
         errs.append(float(np.max(np.abs(xm - ref)) / np.max(np.abs(ref))))
     log(f"[bench] batched MIXED graph: {msm:.3f} ms, max member rel err {max(errs):.2e}")
-    # every member at once, plain FP64 sync-free (heuristic lanes), one launch
-    bt.tune(I.PREC_F64, tune=False)
-    bt.solve(I.PREC_F64, db, dx, graph=False)
-    I.synchronize()
-    timer = I.Timer()
-    timer.start()
-    for _ in range(max(1, args.steps // 4)):
+    ms64b = ms_seq = None
+    if not args.no_extras:
+        # every member at once, plain FP64 sync-free (heuristic lanes), one launch
+        bt.tune(I.PREC_F64, tune=False)
         bt.solve(I.PREC_F64, db, dx, graph=False)
-    ms64b = timer.stop() / max(1, args.steps // 4)
-    assert bt.check() == 0
-    # sequential plain FP64: one system at a time, each its own matrix and launch
-    mats = [I.allocate_memory(sm.ia, sm.ja, sm.a) for sm in systems]
-    bufs = [(I.DeviceArray.from_numpy(sm.b), I.DeviceArray(sm.n, np.float64)) for sm in systems]
-    I.solve(mats[0], None, I.PREC_F64, bufs[0][0], bufs[0][1], kernel=I.KERNEL_SIMPLE)
-    I.synchronize()
-    timer = I.Timer()
-    timer.start()
-    for mt, (bb, xx) in zip(mats, bufs):
-        I.solve(mt, None, I.PREC_F64, bb, xx, kernel=I.KERNEL_SIMPLE)
-    ms_seq = timer.stop()
-    for mt in mats:
-        assert I.check_watchdog(mt) == 0
-        I.free_device_analysis(mt, None)
-    for bb, xx in bufs:
-        bb.free()
-        xx.free()
-    log(f"[bench] FP64 batched one launch: {ms64b:.3f} ms; FP64 sequential: {ms_seq:.1f} ms")
+        I.synchronize()
+        timer = I.Timer()
+        timer.start()
+        for _ in range(max(1, args.steps // 4)):
+            bt.solve(I.PREC_F64, db, dx, graph=False)
+        ms64b = timer.stop() / max(1, args.steps // 4)
+        assert bt.check() == 0
+        # sequential plain FP64: one system at a time, each its own matrix and launch
+        mats = [I.allocate_memory(sm.ia, sm.ja, sm.a) for sm in systems]
+        bufs = [(I.DeviceArray.from_numpy(sm.b), I.DeviceArray(sm.n, np.float64)) for sm in systems]
+        I.solve(mats[0], None, I.PREC_F64, bufs[0][0], bufs[0][1], kernel=I.KERNEL_SIMPLE)
+        I.synchronize()
+        timer = I.Timer()
+        timer.start()
+        for mt, (bb, xx) in zip(mats, bufs):
+            I.solve(mt, None, I.PREC_F64, bb, xx, kernel=I.KERNEL_SIMPLE)
+        ms_seq = timer.stop()
+        for mt in mats:
+            assert I.check_watchdog(mt) == 0
+            I.free_device_analysis(mt, None)
+        for bb, xx in bufs:
+            bb.free()
+            xx.free()
+        log(f"[bench] FP64 batched one launch: {ms64b:.3f} ms; FP64 sequential: {ms_seq:.1f} ms")
+    if args.calibrate_copy:
+        calibrate_copy(I)
     bytes_jac_rows = sum(sm.nnz for m, sm in enumerate(systems) if kinds[m] == I.KERNEL_JACOBI)
     # roofline of the dominant kernel: the kind with the larger summed time
     jac_n = sum(sm.n for m, sm in enumerate(systems) if kinds[m] == I.KERNEL_JACOBI)
     sf_n, sf_nnz = rows - jac_n, nnz - bytes_jac_rows
     t_sw, n_sw = per_kind["sweeps"]
     t_sf, n_sf = per_kind["sync-free"]
+    traffic = None
     if jc is not None and t_sw >= t_sf and n_sw > 0:
         b_launch = bytes_jacobi_sweeps(jac_n, bytes_jac_rows, 1, jc.sweeps, jc.outer, jc.sweeps_next)
         k_ms = t_sw / n_sw
+        if jc.variant == 2:
+            traffic = profiled_traffic(f"k_jac_bsweep<{1024 * max(1, jc.lanes)}>@workload5")
         kname = (f"Jacobi {'block ' if jc.variant == 2 else ''}sweep (FP32) over the {kinds.count(I.KERNEL_JACOBI)} "
                  f"Jacobi members, inside FP64 refinement")
         note = ("effective bandwidth: the Jacobi members' arrays stay in the 256 MB Infinity Cache across a "
@@ -342,10 +362,10 @@ def run_batch(args, I, O) -> int:
             "schedule": dict(zip(("serial_halves", "sync_free_members_as_one_analysed_matrix"),
                                  (bool(v) for v in bt.schedule()))),
             "rel_err_mixed_max_member": max(errs),
-            "f64_batched_one_launch_ms": round(ms64b, 3),
-            "f64_sequential_ms": round(ms_seq, 2),
-            "speedup_vs_sequential_f64": round(ms_seq / msm, 2),
-            "speedup_vs_batched_f64": round(ms64b / msm, 3),
+            "f64_batched_one_launch_ms": None if ms64b is None else round(ms64b, 3),
+            "f64_sequential_ms": None if ms_seq is None else round(ms_seq, 2),
+            "speedup_vs_sequential_f64": None if ms_seq is None else round(ms_seq / msm, 2),
+            "speedup_vs_batched_f64": None if ms64b is None else round(ms64b / msm, 3),
             "kernel_ms_sum_per_solve": round(kms / max(args.steps, 1), 4),
             "kernel_launches_per_solve": launches // max(args.steps, 1),
             "sweeps_ms_per_solve": round(t_sw / max(args.steps, 1), 4),
@@ -357,7 +377,7 @@ def run_batch(args, I, O) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": kname,
             "kernel_ms": round(k_ms, 4),
             "bytes_per_launch": int(b_launch),
@@ -629,7 +649,9 @@ def main() -> int:
     else:
         b_launch = b_method = bytes_mixed_fused(n, nnz, k)
         kname = "sync-free fused mixed-precision solve"
-        key = (f"fused-mixed[kernel={cm.kernel},lanes={cm.lanes},wg={cm.wg},variant={cm.variant},k={k}]"
+        # multirow takes its lanes per row from the analysis: lanes is not part of its key
+        kl = 0 if cm.kernel == I.KERNEL_MULTIROW else cm.lanes
+        key = (f"fused-mixed[kernel={cm.kernel},lanes={kl},wg={cm.wg},variant={cm.variant},k={k}]"
                f"@workload{args.workload}")
         traffic = profiled_traffic(key)
     achieved = b_launch / (kmsm * 1e-3) / 1e9
@@ -694,14 +716,7 @@ def main() -> int:
     dx.free()
     I.free_device_analysis(mat, info)
     if args.calibrate_copy:
-        src = I.DeviceArray(32 << 20, np.float64)                       # 256 MiB
-        dst = I.DeviceArray(32 << 20, np.float64)
-        for _ in range(5):
-            I.copy_device(dst, src, 256 << 20)
-        I.synchronize()
-        src.free()
-        dst.free()
-        log("[bench] calibration: 5 k_copy16 copies of 262144 KiB (read 262144 KiB, write 262144 KiB each)")
+        calibrate_copy(I)
     return 0
 
 

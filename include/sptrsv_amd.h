@@ -341,7 +341,9 @@ int csr_L_solve_mixed(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b,
  * 64-bit words (32 MiB, held by every GPU the library runs on until the
  * process ends) instead of the matrix's workspace.  It calls no hipMalloc,
  * hipFree or other allocating routine.  n * k above the maximum is
- * SPTRSV_E_CAPACITY.  Sync-free kernels only: a configuration that resolves
+ * SPTRSV_E_CAPACITY; a MULTIROW solve that splits long rows (one right-hand
+ * side) also keeps its segment partials there, after the published words,
+ * and must fit both.  Sync-free kernels only: a configuration that resolves
  * to JACOBI is SPTRSV_E_INVALID_ARG, as is a handle stored in reversed order
  * (allocate_memory_upper, or the transpose of a lower-triangular handle).
  * Each call re-initialises the scratch it uses.  Every call on one GPU
@@ -415,6 +417,13 @@ int sptrsv_batch_choice(const sptrsv_batch_t* bt, int m, sptrsv_config_t* cfg);
  * back to the batch order); 0 launches them as ticketed tasks. */
 int sptrsv_batch_schedule(const sptrsv_batch_t* bt, int* serial, int* separate);
 int sptrsv_batch_set_schedule(sptrsv_batch_t* bt, int serial, int separate);
+/* Members sharing a structure (same n, row lengths and columns; values may
+ * differ): *n_patterns distinct structures, and pattern_of[m] (nmat entries,
+ * may be NULL) the first member with m's structure.  sptrsv_batch_tune times
+ * the sync-free launch once per structure and gives its lanes choice to every
+ * member of that structure (the work, hence the time, depends on the
+ * structure only); value-dependent choices (Jacobi) stay per member. */
+int sptrsv_batch_patterns(const sptrsv_batch_t* bt, int* n_patterns, int* pattern_of);
 /* sptrsv_check for the batch: watchdog, or a Jacobi member that missed its
  * certified bound (each member is certified separately). */
 int sptrsv_batch_check(sptrsv_batch_t* bt, hipStream_t stream);

@@ -129,6 +129,7 @@ _PROTOS = {
     "sptrsv_batch_choice": (c_int, [c_void_p, c_int, POINTER(Config)]),
     "sptrsv_batch_check": (c_int, [c_void_p, c_void_p]),
     "sptrsv_batch_schedule": (c_int, [c_void_p, _P_INT, _P_INT]),
+    "sptrsv_batch_patterns": (c_int, [c_void_p, _P_INT, c_void_p]),
     "sptrsv_batch_set_schedule": (c_int, [c_void_p, c_int, c_int]),
     "sptrsv_batch_destroy": (None, [c_void_p]),
     "csr_L_rocsparse_create": (c_int, [POINTER(SpMat), POINTER(c_void_p), POINTER(c_float)]),

@@ -40,6 +40,7 @@ int solve_checked(sp_mat_t* m, dfr_analysis_info_t* info, int precision, const s
                   const void* b, void* x, int n, int k, int refine_steps, hipStream_t s);
 int convert_values_to_f32(sp_mat_t* m);
 int reserve_part(sp_mat_t* m, const dfr_analysis_info_t* info);
+unsigned long long fnv1a(const void* p, size_t n, unsigned long long h);
 void jacobi_host_tiles(const int* ia, int r_begin, int r_end, long long row_base, long long entry_base,
                        int window, std::vector<int>& out);
 int jacobi_set_tiles(sp_mat_t* m, const std::vector<int>* tiles4, const std::vector<int>* tile_member,
@@ -131,6 +132,8 @@ struct sptrsv_batch {
     std::vector<long long> ent_off;      // nmat+1: first entry of each matrix
     std::vector<int> max_row;            // per matrix: longest row
     std::vector<double> chain;           // per matrix: fraction of rows i > 0 that depend on row i - 1
+    std::vector<int> pattern;            // per matrix: first matrix with the same structure (n, ia, ja)
+    int n_patterns = 0;
     // MIXED: the sync-free matrices as one matrix of their own, with its own
     // level analysis and tuned configuration; its rows are gathered from and
     // scattered back to the batch order (null: the ticketed task launch)
@@ -185,6 +188,45 @@ void drop_batch_graphs(sptrsv_batch_t* bt) {
     (void)hipDeviceSynchronize();
     for (auto& g : bt->graphs) (void)hipGraphExecDestroy(g.second);
     bt->graphs.clear();
+}
+
+// Members with the same structure (n, row lengths, columns): pattern[m] is
+// the first such member.  A structure hash finds candidates; equal arrays
+// confirm them.
+void find_patterns(sptrsv_batch_t* bt, const int* n, const int* const* ia, const int* const* ja) {
+    const int nmat = (int)bt->lanes.size();
+    bt->pattern.assign(nmat, 0);
+    bt->n_patterns = 0;
+    std::map<unsigned long long, std::vector<int>> by_hash;
+    auto same = [&](int p, int m) {
+        if (n[p] != n[m]) return false;
+        const int bp = n[p] > 0 ? ia[p][0] : 0, bm = n[m] > 0 ? ia[m][0] : 0;
+        for (int i = 0; i <= n[m]; ++i)
+            if (ia[p][i] - bp != ia[m][i] - bm) return false;
+        const int nz = n[m] > 0 ? ia[m][n[m]] - bm : 0;
+        return nz == 0 || std::memcmp(ja[p] + bp, ja[m] + bm, sizeof(int) * (size_t)nz) == 0;
+    };
+    for (int m = 0; m < nmat; ++m) {
+        unsigned long long h = fnv1a(&n[m], sizeof(int), 1469598103934665603ull);
+        const int base = n[m] > 0 ? ia[m][0] : 0;
+        for (int i = 1; i <= n[m]; ++i) {
+            const int len = ia[m][i] - ia[m][i - 1];
+            h = fnv1a(&len, sizeof(int), h);
+        }
+        if (n[m] > 0) h = fnv1a(ja[m] + base, sizeof(int) * (size_t)(ia[m][n[m]] - base), h);
+        std::vector<int>& cand = by_hash[h];
+        int rep = m;
+        for (int p : cand)
+            if (same(p, m)) {
+                rep = p;
+                break;
+            }
+        if (rep == m) {
+            cand.push_back(m);
+            ++bt->n_patterns;
+        }
+        bt->pattern[m] = rep;
+    }
 }
 
 int lanes_rule(int avg) {
@@ -489,6 +531,7 @@ int sptrsv_batch_create(int nmat, const int* n, const int* const* ia, const int*
         entry_base += nz;
     }
     bt->ent_off[nmat] = entry_base;
+    find_patterns(bt, n, ia, ja);
     st = build_tasks(bt);
     if (st == SPTRSV_OK) st = sptrsv_reserve(&bt->mat, 1);
     if (st == SPTRSV_OK) st = install_jacobi(bt);
@@ -521,6 +564,13 @@ int sptrsv_batch_choice(const sptrsv_batch_t* bt, int m, sptrsv_config_t* cfg) {
     if (!bt || !cfg || m < 0 || m >= (int)bt->lanes.size()) return fail(SPTRSV_E_INVALID_ARG, "bad index");
     *cfg = sptrsv_config_t{SPTRSV_KERNEL_SIMPLE, bt->lanes[m], kBatchWG, 1};
     if (bt->choice[m] == SPTRSV_KERNEL_JACOBI) *cfg = bt->jac_cfg;
+    return SPTRSV_OK;
+}
+
+int sptrsv_batch_patterns(const sptrsv_batch_t* bt, int* n_patterns, int* pattern_of) {
+    if (!bt || !n_patterns) return fail(SPTRSV_E_INVALID_ARG, "null batch or count");
+    *n_patterns = bt->n_patterns;
+    if (pattern_of) std::copy(bt->pattern.begin(), bt->pattern.end(), pattern_of);
     return SPTRSV_OK;
 }
 
@@ -627,7 +677,19 @@ int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
         else SPT_TRY(st);
     }
     std::vector<int> slow;
+    std::vector<char> lanes_tuned(nmat, 0);
     for (int m = 0; m < nmat; ++m) {
+        // a member with an earlier member's structure takes that member's
+        // sync-free timing and lanes (the work, hence the time, depends on
+        // the structure only)
+        const int p = bt->pattern[m];
+        if (p != m && lanes_tuned[p]) {
+            bt->lanes[m] = bt->lanes[p];
+            bt->t_sync[m] = bt->t_sync[p];
+            lanes_tuned[m] = 1;
+            if (jall > 0 && bt->delta[m] > 0 && bt->t_sync[m] > 0.5f * jall) slow.push_back(m);
+            continue;
+        }
         const int ref = lanes_rule(bt->avg[m]);
         float best = 0;
         SPT_TRY(time_member(bt, ts, SPTRSV_PREC_MIXED, m, ref, 1, &best));
@@ -644,6 +706,7 @@ int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
                     bt->lanes[m] = L;
                 }
             }
+            lanes_tuned[m] = 1;
         }
         bt->t_sync[m] = best;
     }
@@ -764,6 +827,11 @@ int sptrsv_batch_tune(sptrsv_batch_t* bt, int precision, int tune) {
     SPT_TRY(ts.init(bt->mat.nr));
     if (precision == SPTRSV_PREC_MIXED) return tune_mixed(bt, ts);
     for (int m = 0; m < nmat; ++m) {
+        if (bt->pattern[m] != m) {  // timed once per structure
+            bt->lanes[m] = bt->lanes[bt->pattern[m]];
+            bt->t_sync[m] = bt->t_sync[bt->pattern[m]];
+            continue;
+        }
         const int ref = lanes_rule(bt->avg[m]);
         float best = 1e30f;
         for (int L = std::max(1, ref / 2); L <= std::min(64, ref * 2); L <<= 1) {

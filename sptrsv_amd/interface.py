@@ -523,6 +523,13 @@ class Batch:
         """Status of the last solves (0, E_WATCHDOG or E_NOT_CONVERGED) without raising."""
         return L.lib().sptrsv_batch_check(self.h, stream)
 
+    def patterns(self) -> tuple[int, np.ndarray]:
+        """(distinct structures, first member with each member's structure)."""
+        cnt = c_int()
+        of = np.empty(self.nmat, np.int32)
+        check(L.lib().sptrsv_batch_patterns(self.h, byref(cnt), of.ctypes.data), "sptrsv_batch_patterns")
+        return cnt.value, of
+
     def schedule(self) -> tuple[int, int]:
         """(serial, separate) of a MIXED batched solve: halves one after the
         other; sync-free members solved as one analysed matrix of their own."""

@@ -435,6 +435,44 @@ def test_batch_graph_replays(device):
     bt.close()
 
 
+def test_batch_patterns_share_tuning(device):
+    """Members on the same structure with other values: found as one
+    structure, timed once, given the same lanes, and each solved right."""
+    import types
+    rng = np.random.default_rng(17)
+    base = [I.generate_batch_member(m, 500, 6000, seed=5) for m in range(3)]
+
+    def revalued(s):
+        a = s.a.copy()
+        diag = s.ia[1:] - 1
+        off = np.ones(len(a), bool)
+        off[diag] = False
+        a[off] *= rng.uniform(0.5, 1.5, off.sum())
+        rows = np.repeat(np.arange(s.n), np.diff(s.ia))
+        a[diag] = 1.0 + np.bincount(rows[off], weights=np.abs(a[off]), minlength=s.n)
+        return types.SimpleNamespace(n=s.n, ia=s.ia, ja=s.ja.copy(), a=a, b=rng.uniform(-1, 1, s.n))
+
+    systems = [base[0], base[1], revalued(base[0]), revalued(base[1]), revalued(base[0]), base[2]]
+    bt = I.Batch(systems)
+    count, of = bt.patterns()
+    assert count == 3 and of.tolist() == [0, 1, 0, 1, 0, 5]
+    B = np.concatenate([s.b for s in systems])
+    db = I.DeviceArray.from_numpy(B)
+    dx = I.DeviceArray(bt.rows, np.float64)
+    for prec, tol in ((I.PREC_F64, 1e-12), (I.PREC_MIXED, 1e-10)):
+        bt.tune(prec, tune=True)
+        lanes = [bt.choice(m).lanes for m in range(len(systems))]
+        assert lanes[0] == lanes[2] == lanes[4] and lanes[1] == lanes[3], lanes
+        bt.solve(prec, db, dx, graph=True)
+        I.synchronize()
+        assert bt.check() == 0
+        x = dx.numpy()
+        for m, s in enumerate(systems):
+            lo, hi = bt.row_off[m], bt.row_off[m + 1]
+            assert relerr(x[lo:hi], O.fwd_f64(s.ia, s.ja, s.a, s.b)) <= tol, (prec, m)
+    bt.close()
+
+
 def test_rocsparse_baseline(device):
     s = I.generate(2, 50000)
     ref = O.fwd_f64(s.ia, s.ja, s.a, s.b)
