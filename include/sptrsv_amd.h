@@ -272,7 +272,12 @@ int csr_L_solve_multirow_f32(sp_mat_t* mat, dfr_analysis_info_t* info, const flo
  * 64/lanes rows per wave (rows of a wave may depend on each other, natural
  * order shares results through LDS), 1 is the lean path where no wave holds
  * two dependent rows (SIMPLE/ORDER: one row per wave; MULTIROW: its
- * single-level waves), one right-hand side only; cu_waves caps the waves
+ * single-level waves), one right-hand side only; 2 is a persistent walk of
+ * the same rows (block-claimed, the grid capped by cu_waves); 3 is the
+ * in-degree walk (SIMPLE/ORDER, one right-hand side): each row waits on a
+ * counter of its unsolved dependencies, which solved rows decrement through
+ * the column pattern, then folds its entries exactly as variant 1 does (same
+ * results, bit for bit); cu_waves caps the waves
  * resident per CU (0 = hardware limit): every resident wave polls, and past a
  * few thousand polling waves the dependency hand-off latency grows several
  * fold, so deep dependency chains run fastest with a small cap.  With lanes

@@ -574,7 +574,7 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
         A.dyn_lds = dyn_lds;
         hipLaunchKernelGGL((k_prepare<uint32_t>), dim3(grid_for(n)), dim3(256), 0, s,
                            reinterpret_cast<uint32_t*>(info->level), (size_t)n, 0u, pv->counters, nullptr,
-                           (size_t)0);
+                           (size_t)0, nullptr, nullptr, 0);
         SPT_HIP(hipGetLastError());
         int u = (pv->max_row > 2 * lanes) ? 4 : 1;
         hipError_t e = launch_solve<PolLevel, 1>(kind, 1, wg, u, A, s);

@@ -193,7 +193,7 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     if (K > 1) {
         // several RHS: the general engine, ticketed (256 threads) or
         // persistent (256 or 1024 threads)
-        if (out->kernel != SPTRSV_KERNEL_JACOBI && out->variant != 2) out->variant = 0;
+        if (out->kernel != SPTRSV_KERNEL_JACOBI && out->variant != 2 && out->variant != 3) out->variant = 0;
         if (!(out->variant == 2 && out->wg == 1024)) out->wg = 256;
     }
     if (out->cu_waves < 0) out->cu_waves = 0;
@@ -216,6 +216,11 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
         return fail(SPTRSV_E_INVALID_ARG, "unknown kernel");
     if (out->kernel == SPTRSV_KERNEL_JACOBI && precision != SPTRSV_PREC_MIXED)
         return fail(SPTRSV_E_INVALID_ARG, "JACOBI is the bulk solver of the MIXED path only");
+    if (out->kernel != SPTRSV_KERNEL_JACOBI && (out->variant < 0 || out->variant > 3))
+        return fail(SPTRSV_E_INVALID_ARG, "variant must be 0..3");
+    if (out->variant == 3 && out->kernel != SPTRSV_KERNEL_JACOBI &&
+        (K > 1 || out->kernel == SPTRSV_KERNEL_MULTIROW))
+        return fail(SPTRSV_E_INVALID_ARG, "the in-degree walk (variant 3) takes SIMPLE or ORDER and one right-hand side");
     return SPTRSV_OK;
 }
 
@@ -323,6 +328,59 @@ static size_t split_part_words(const dfr_analysis_info_t* info, const sptrsv_con
     return sg ? 2 * (size_t)sg->n_segs : 0;
 }
 
+// ---- in-degree walk: the off-diagonal pattern by column ------------------------------
+// Off-diagonal entry q of row i is the (q - i)-th off-diagonal entry overall
+// (each earlier row has exactly one diagonal, last), so keys and rows land
+// compacted without a scan.
+__global__ void k_dep_keys(const int* __restrict__ ia, const int* __restrict__ ja, int n, int* __restrict__ keys,
+                           int* __restrict__ rows, int* __restrict__ cnt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int q = ia[i]; q < ia[i + 1] - 1; ++q) {
+            keys[q - i] = ja[q];
+            rows[q - i] = i;
+            atomicAdd(&cnt[ja[q]], 1);
+        }
+}
+
+// Counters and dependents of every row (a stable sort by column keeps each
+// column's rows ascending).  Setup only: allocates, before any capture.
+static int ensure_indeg(sp_mat_t* m) {
+    MatPriv* pv = priv_of(m);
+    if (pv->indeg) return SPTRSV_OK;
+    const int n = m->nr;
+    const long long off = (long long)m->nnz - n;  // every row holds its diagonal
+    const size_t no = (size_t)std::max<long long>(off, 1);
+    int *keys = nullptr, *tk = nullptr, *tv = nullptr;
+    int st = SPTRSV_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && st == SPTRSV_OK) st = hip_fail(e, what, __FILE__, __LINE__);
+    };
+    chk(hipMalloc(&pv->dep_ptr, sizeof(int) * ((size_t)n + 1)), "hipMalloc dependents");
+    chk(hipMalloc(&pv->dep_row, sizeof(int) * no), "hipMalloc dependents");
+    chk(hipMalloc(&keys, sizeof(int) * no), "hipMalloc keys");
+    chk(hipMalloc(&tk, sizeof(int) * no), "hipMalloc keys");
+    chk(hipMalloc(&tv, sizeof(int) * no), "hipMalloc keys");
+    if (st == SPTRSV_OK) chk(hipMemset(pv->dep_ptr, 0, sizeof(int) * ((size_t)n + 1)), "clear counts");
+    if (st == SPTRSV_OK && n > 0) {
+        hipLaunchKernelGGL(k_dep_keys, dim3(stream_grid((size_t)n)), dim3(256), 0, nullptr, m->ia, m->ja, n, keys,
+                           pv->dep_row, pv->dep_ptr);
+        chk(hipGetLastError(), "dependents keys");
+        if (st == SPTRSV_OK && off > 0) st = radix_sort_pairs(keys, pv->dep_row, tk, tv, (int)off, n - 1, nullptr);
+        if (st == SPTRSV_OK) st = exclusive_scan(pv->dep_ptr, pv->dep_ptr, (long long)n + 1, nullptr);
+    }
+    if (st == SPTRSV_OK) chk(hipMalloc(&pv->indeg, sizeof(int) * (size_t)std::max(n, 1)), "hipMalloc counters");
+    if (st == SPTRSV_OK) chk(hipDeviceSynchronize(), "dependents sync");
+    for (int* q : {keys, tk, tv})
+        if (q) (void)hipFree(q);
+    if (st != SPTRSV_OK) {
+        for (int** q : {&pv->indeg, &pv->dep_ptr, &pv->dep_row}) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
+    }
+    return st;
+}
+
 // The matrix's partials workspace sized for `info`'s segment table (before a
 // capture, where no allocation may happen).
 int reserve_part(sp_mat_t* m, const dfr_analysis_info_t* info) {
@@ -374,6 +432,14 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     int kind = c.kernel == SPTRSV_KERNEL_ORDER ? KK_ORDER
                : c.kernel == SPTRSV_KERNEL_MULTIROW ? KK_MULTIROW : KK_SIMPLE;
     if (c.variant == 1 && K == 1) kind += KK_FAST_SIMPLE - KK_SIMPLE;
+    if (c.variant == 3) {
+        // in-degree walk (one right-hand side; SIMPLE or ORDER, checked when resolved)
+        SPT_TRY(ensure_indeg(m));
+        kind = c.kernel == SPTRSV_KERNEL_ORDER ? KK_INDEG_ORDER : KK_INDEG_SIMPLE;
+        A.indeg = pv->indeg;
+        A.dep_ptr = pv->dep_ptr;
+        A.dep_row = pv->dep_row;
+    }
     if (c.variant == 2) {
         kind += KK_PERSIST_SIMPLE - KK_SIMPLE;
         // persistent: the wave count itself is the residency cap
@@ -396,7 +462,8 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     size_t words = (size_t)m->nr * k;
     using word = typename P::word;
     hipLaunchKernelGGL((k_prepare<word>), dim3(stream_grid(words)), dim3(256), 0, s,
-                       static_cast<word*>(pub), words, P::kUnsolved, pv->counters, A.part, part_words);
+                       static_cast<word*>(pub), words, P::kUnsolved, pv->counters, A.part, part_words, m->ia,
+                       A.indeg, A.indeg ? m->nr : 0);
     SPT_HIP(hipGetLastError());
     int u = c.window > 0 ? c.window : window_for(pv, (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW) ? 32 : c.lanes);
     const KtPair ev = timed ? kernel_timer_pair(s) : KtPair{};
@@ -687,6 +754,9 @@ int sptrsv_solve_mixed_noalloc(sp_mat_t* mat, dfr_analysis_info_t* info, const s
         return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve runs the sync-free kernels only "
                                           "(JACOBI keeps per-matrix workspaces); name SIMPLE, ORDER or MULTIROW");
     c.fused = 1;
+    if (c.variant == 3)
+        return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve does not run the in-degree walk (its "
+                                          "counters are a per-matrix workspace)");
     // the published words, then (split long rows) the segment partials from
     // the next 256-byte boundary
     const size_t pub_words = (size_t)n * k;

@@ -63,6 +63,11 @@ struct MatPriv {
     size_t pub_words = 0;
     uint64_t* part = nullptr;    // published partial sums of long-row segments, 2 per segment (device)
     size_t part_words = 0;
+    // in-degree walk: per-row counters and the off-diagonal pattern by column
+    // (built on first use; structure only, so new values keep them)
+    int* indeg = nullptr;
+    int* dep_ptr = nullptr;
+    int* dep_row = nullptr;
     double* resid = nullptr;     // two-pass refinement buffers (device)
     float* corr = nullptr;
     float* b32 = nullptr;

@@ -105,6 +105,8 @@ void free_mat(sp_mat_t* m) {
         if (pv->counters) (void)hipFree(pv->counters);
         if (pv->pub) (void)hipFree(pv->pub);
         if (pv->part) (void)hipFree(pv->part);
+        for (int* q : {pv->indeg, pv->dep_ptr, pv->dep_row})
+            if (q) (void)hipFree(q);
         if (pv->b32) (void)hipFree(pv->b32);
         if (pv->corr) (void)hipFree(pv->corr);
         if (pv->resid) (void)hipFree(pv->resid);

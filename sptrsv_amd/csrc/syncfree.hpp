@@ -70,6 +70,11 @@ struct SolveArgs {
     const int* seg_hi;
     const int* row_seg;     // first segment of each row, -1 for none
     uint64_t* part;         // partial sums, 2 words per segment (all-ones = not published)
+    // in-degree walk (variant 3): per-row count of unsolved dependencies and
+    // the dependents of each row (the off-diagonal pattern by column)
+    int* indeg;
+    const int* dep_ptr;     // n + 1
+    const int* dep_row;     // rows i > j with an entry in column j, ascending
 };
 
 template <int K>
@@ -896,6 +901,76 @@ __global__ __launch_bounds__(WG) void k_fast_multirow(SolveArgs A) {
     fast_rows<P, U, true>(A, L, lane, row);
 }
 
+// ---- in-degree walk (one right-hand side) ------------------------------------------
+// The column-oriented schedule of sync-free SpTRSV: every row keeps a count
+// of its unsolved dependencies; a solved row decrements the count of each row
+// that depends on it (its column's pattern), and a row starts folding once
+// its count is zero.  A waiting row polls one word, its counter, instead of
+// one per pending entry.  The values are still pulled in storage order
+// through fold_entries (whose polls then succeed at once: a counter reaches
+// zero only after every dependency stored its value, and a value read early
+// shows the unsolved pattern and is read again), so the results are
+// bit-identical to the lean walk with the same lanes and window -- no
+// floating-point atomics, unlike the push-sum form of this schedule.  Rows
+// are taken in ticket order, so every decrement a row waits for comes from a
+// running wave.
+template <class P, int U>
+__device__ __forceinline__ void indeg_row(const SolveArgs& A, int L, int lane, int row) {
+    using word = typename P::word;
+    using Acc = typename P::Acc;
+    word* gpub = static_cast<word*>(A.pub);
+    const int e = lane & (L - 1);
+    const bool live = lane < L;
+    const int rs = A.ia[row], dpos = A.ia[row + 1] - 1;
+    const size_t xi = (size_t)row;
+    Acc acc = (live && e == 0) ? P::seed(A, xi) : P::zero();
+    typename P::Diag dg{};
+    if (live && e == 0) dg = P::diag(A, dpos);
+    if (A.trace && live && e == 0) A.trace[3 * xi] = wall_clock64();
+    const long long t0 = wall_clock64();
+    bool expired = false;
+    int spins = 0;
+    // wait until every dependency has been solved (lane 0 polls for the wave)
+    while (true) {
+        int c = 0;
+        if (lane == 0) c = poll_global(&A.indeg[row]);
+        if (__shfl(c, 0) <= 0) break;
+        if (++spins == 512) {
+            spins = 0;
+            if (wall_clock64() - t0 > A.wd_ticks) {
+                expired = true;  // fold_entries then poisons what is missing
+                if (lane == 0) atomicOr(A.err, 1);
+                break;
+            }
+        }
+    }
+    fold_entries<P, U>(A, L, e, lane, live, rs, dpos, acc, t0, expired, spins);
+    const Acc t = group_reduce<P, 1>(acc, L);
+    if (live && e == 0) {
+        if (A.trace) A.trace[3 * xi + 1] = wall_clock64();
+        typename P::Out o{};
+        const word w = P::finish(A, dg, t, o);
+        P::emit(A, xi, o);
+        if (A.trace) A.trace[3 * xi + 2] = wall_clock64();
+        publish_global(&gpub[xi], w);
+    }
+    // hand the row to its dependents
+    for (int q = A.dep_ptr[row] + lane; q < A.dep_ptr[row + 1]; q += kWave)
+        __hip_atomic_fetch_add(&A.indeg[A.dep_row[q]], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One row per wave in natural (ORDERED = false) or level order.
+template <class P, int WG, int U, bool ORDERED>
+__global__ __launch_bounds__(WG) void k_indeg(SolveArgs A) {
+    __shared__ int s_base;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_base = atomicAdd(A.ticket, 1) * (WG / kWave);
+    __syncthreads();
+    const int pos = s_base + (tid >> 6);
+    if (pos >= A.n) return;  // whole wave
+    indeg_row<P, U>(A, A.L, tid & (kWave - 1), ORDERED ? A.iorder[pos] : pos);
+}
+
 // ---- persistent lean path --------------------------------------------------------
 // A fixed number of workgroups (the residency cap) walks the work items:
 // rows in natural order (MODE 0), rows in iorder (MODE 1) or multirow waves of
@@ -1003,14 +1078,17 @@ __global__ __launch_bounds__(WG) void k_persist_rows(SolveArgs A) {
 }
 
 // Fill the published words with the unsolved pattern, the segment partials
-// (if any) with all-ones, and reset the ticket.
+// (if any) with all-ones, the in-degree counters (if any) with each row's
+// off-diagonal count, and reset the ticket.
 template <class T>
-__global__ void k_prepare(T* pub, size_t count, T unsolved, int* ticket, uint64_t* part, size_t part_count) {
+__global__ void k_prepare(T* pub, size_t count, T unsolved, int* ticket, uint64_t* part, size_t part_count,
+                          const int* ia, int* indeg, int n_indeg) {
     const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     if (i0 == 0) *ticket = 0;
     for (size_t i = i0; i < count; i += stride) pub[i] = unsolved;
     for (size_t i = i0; i < part_count; i += stride) part[i] = kUnsolved64;
+    for (size_t i = i0; i < (size_t)n_indeg; i += stride) indeg[i] = ia[i + 1] - ia[i] - 1;
 }
 
 // ---- host-side launch ----------------------------------------------------------
@@ -1024,6 +1102,8 @@ enum KernelKind {
     KK_PERSIST_SIMPLE = 7,    // persistent walk over rows (lean for K == 1)
     KK_PERSIST_ORDER = 8,     // persistent walk over iorder (lean for K == 1)
     KK_PERSIST_MULTIROW = 9,  // persistent walk over multirow waves (lean for K == 1)
+    KK_INDEG_SIMPLE = 10,     // in-degree walk, natural order (K == 1)
+    KK_INDEG_ORDER = 11,      // in-degree walk, level order (K == 1)
 };
 
 // Launch of a persistent kernel with A.persist_grid workgroups, clamped to
@@ -1058,7 +1138,7 @@ hipError_t launch_k(int kind, const SolveArgs& A, hipStream_t s) {
     unsigned grid;
     if (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW)
         grid = ceil_div(A.items ? A.n_items : A.n_waves, WG / kWave);
-    else if (kind == KK_FAST_SIMPLE || kind == KK_FAST_ORDER)
+    else if (kind == KK_FAST_SIMPLE || kind == KK_FAST_ORDER || kind == KK_INDEG_SIMPLE || kind == KK_INDEG_ORDER)
         grid = ceil_div(A.n, WG / kWave);
     else
         grid = ceil_div(A.n, WG >> (A.lgL + kLog2<K>()));
@@ -1080,6 +1160,12 @@ hipError_t launch_k(int kind, const SolveArgs& A, hipStream_t s) {
                 return launch_persist<P, WG, U, 1>(A, s);
             case KK_PERSIST_MULTIROW:
                 return launch_persist<P, WG, U, 2>(A, s);
+            case KK_INDEG_SIMPLE:
+                hipLaunchKernelGGL((k_indeg<P, WG, U, false>), dim3(grid), dim3(WG), A.dyn_lds, s, A);
+                return hipGetLastError();
+            case KK_INDEG_ORDER:
+                hipLaunchKernelGGL((k_indeg<P, WG, U, true>), dim3(grid), dim3(WG), A.dyn_lds, s, A);
+                return hipGetLastError();
             default:
                 break;
         }
