@@ -406,40 +406,49 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const f
     const int rpp = kTileThreads / G2;
     const int npass = (rows + rpp - 1) / rpp;
     const int e = tid % G2;
-    // base: the off-tile sums
-    for (int ps = 0; ps < npass; ++ps) {
-        const int lr = ps * rpp + tid / G2;
-        const bool live = lr < rows;
-        float sm = 0.0f;
-        if (live) {
-            const int q1 = s_ia[lr + 1] - 1;  // diagonal excluded
-            for (int q = s_ia[lr] + e; q < q1; q += G2)
-                if (s_c[q] < 0) sm += s_v[q];
-        }
-        sm = jac_group_sum(sm, G2, 1);
-        if (live && e == 0) s_b[lr] -= sm;
-    }
     // inner iterations on the diagonal block.  With one pass of rows per
     // workgroup (the usual case) each lane keeps its first EC in-tile
     // entries in registers across the iterations, so an iteration costs one
     // round of independent LDS gathers.
     constexpr int EC = 8;
     int cur = 0;
-    if (npass == 1) {
-        __syncthreads();
+    if (npass == 1 && inner > 0) {
+        // one pass over the tile gives both the base (the off-tile sums)
+        // and the first inner iterate z^1 = base - N'_in z^0, each summed in
+        // the order of the separate passes (so the results are the same)
         const int lr = tid / G2;
         const bool live = lr < rows;
         const int q0 = live ? s_ia[lr] + e : 0, q1 = live ? s_ia[lr + 1] - 1 : 0;
         int cc[EC];
         float cv[EC];
+        float so = 0.0f, si = 0.0f;
 #pragma unroll
         for (int u = 0; u < EC; ++u) {
             const int q = q0 + u * G2;
-            cc[u] = (q < q1) ? s_c[q] : -1;
-            cv[u] = (cc[u] >= 0) ? s_v[q] : 0.0f;
+            const int c = (q < q1) ? s_c[q] : -1;
+            const float v = (q < q1) ? s_v[q] : 0.0f;
+            if (q < q1 && c < 0) so += v;
+            cc[u] = c;
+            cv[u] = (c >= 0) ? v : 0.0f;
+            si = __fmaf_rn(cv[u], s_z0[max(c, 0)], si);
         }
         const int qrest = q0 + EC * G2;
-        for (int it = 0; it < inner; ++it) {
+        for (int q = qrest; q < q1; q += G2) {
+            const int c = s_c[q];
+            if (c < 0)
+                so += s_v[q];
+            else
+                si = __fmaf_rn(s_v[q], s_z0[c], si);
+        }
+        so = jac_group_sum(so, G2, 1);
+        si = jac_group_sum(si, G2, 1);
+        if (live && e == 0) {
+            const float base = s_b[lr] - so;
+            s_b[lr] = base;
+            s_z1[lr] = base - si;
+        }
+        cur = 1;
+        for (int it = 1; it < inner; ++it) {
             if (it > 0) __syncthreads();
             const float* zc = cur ? s_z1 : s_z0;
             float sm = 0.0f;
@@ -454,6 +463,19 @@ __global__ __launch_bounds__(kTileThreads) void k_jac_bsweep(TileArgs A, const f
             cur ^= 1;
         }
     } else {
+        // base: the off-tile sums
+        for (int ps = 0; ps < npass; ++ps) {
+            const int lr = ps * rpp + tid / G2;
+            const bool live = lr < rows;
+            float sm = 0.0f;
+            if (live) {
+                const int q1 = s_ia[lr + 1] - 1;  // diagonal excluded
+                for (int q = s_ia[lr] + e; q < q1; q += G2)
+                    if (s_c[q] < 0) sm += s_v[q];
+            }
+            sm = jac_group_sum(sm, G2, 1);
+            if (live && e == 0) s_b[lr] -= sm;
+        }
         for (int it = 0; it < inner; ++it) {
             __syncthreads();
             const float* zc = cur ? s_z1 : s_z0;
