@@ -474,6 +474,8 @@ int build_segments(dfr_analysis_info_t* info, sp_mat_t* m) {
     SegTable& t = static_cast<InfoPriv*>(info->priv)->seg;
     free_segments(t);
     if (!info->ibase_row64 || info->n_waves == 0 || m->nr == 0) return SPTRSV_OK;
+    const MatPriv* pv = priv_of(m);
+    if (pv && pv->max_row <= kSegMin + 1) return SPTRSV_OK;  // no row long enough to split
     const int nw = info->n_waves, n = m->nr;
     hipStream_t s = nullptr;
     int* cnt = nullptr;
