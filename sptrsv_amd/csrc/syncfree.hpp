@@ -731,13 +731,21 @@ __device__ __forceinline__ void fold_entries(const SolveArgs& A, int L, int e, i
             got[u] = P::kUnsolved;
         }
         while (true) {
+            // every pending poll of the chunk goes out before the first is
+            // examined, so a turn costs one round trip (a check right after
+            // each load would make the compiler wait for that load alone)
+            word w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                w[u] = P::kUnsolved;
+                if (pend[u]) w[u] = poll_global(&gpub[col[u]]);
+            }
             bool any = false;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (pend[u]) {
-                    word w = poll_global(&gpub[col[u]]);
-                    if (w != P::kUnsolved) {
-                        got[u] = w;
+                    if (w[u] != P::kUnsolved) {
+                        got[u] = w[u];
                         pend[u] = false;
                     } else if (expired) {
                         got[u] = P::kPoison;
