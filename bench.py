@@ -47,9 +47,10 @@ def bytes_f64(n, nnz, k):
     return 4 * (n + 1) + 12 * nnz + 16 * k * n
 
 
-def bytes_mixed_fused(n, nnz, k):
-    """Fused mixed pass: ia + ja + a32 + a64 + b (f64) + x (f64) + packed (x0,d) words."""
-    return 4 * (n + 1) + 16 * nnz + 24 * k * n
+def bytes_mixed_fused(n, nnz, k, reorder=0):
+    """Fused mixed pass: ia + ja + a32 + a64 + b (f64) + x (f64) + packed (x0,d) words
+    (+ the caller's row of each row when a level-ordered copy is walked)."""
+    return 4 * (n + 1) + 16 * nnz + 24 * k * n + (4 * n if reorder else 0)
 
 
 def bytes_method_convention(n, nnz, k):
@@ -100,9 +101,9 @@ PRESETS = {
     2: {"f64": dict(kernel=2, lanes=16, wg=256, fused=1, variant=2, cu_waves=4),
         "mixed": dict(kernel=4, lanes=2, wg=256, fused=1, variant=1, sweeps=12, outer=2, window=16)},
     3: {"f64": dict(kernel=3, lanes=32, wg=1024, fused=1, variant=2, cu_waves=8),
-        "mixed": dict(kernel=3, lanes=32, wg=1024, fused=1, variant=2, cu_waves=8)},
+        "mixed": dict(kernel=3, lanes=32, wg=1024, fused=1, variant=2, cu_waves=8, reorder=1)},
     4: {"f64": dict(kernel=2, lanes=1, wg=1024, fused=1, variant=2, cu_waves=6),
-        "mixed": dict(kernel=2, lanes=1, wg=1024, fused=1, variant=2, cu_waves=12),
+        "mixed": dict(kernel=2, lanes=1, wg=1024, fused=1, variant=2, cu_waves=12, reorder=1),
         "f64_per_rhs": dict(kernel=3, lanes=4, wg=256, fused=1, variant=2, cu_waves=6)},
 }
 
@@ -121,6 +122,8 @@ def cfg_dict(c) -> dict:
             d["claim_rounds"] = c.claim_rounds or 1
         if c.kernel == 3:
             d["split_rows"] = c.split_rows >= 0
+        if c.reorder:
+            d["level_ordered_copy"] = True
     return d
 
 
@@ -647,11 +650,12 @@ def main() -> int:
         kname = f"Jacobi sweep (FP32, {key.split('@')[0]}) inside FP64 refinement"
         traffic = profiled_traffic(key)
     else:
-        b_launch = b_method = bytes_mixed_fused(n, nnz, k)
+        b_launch = b_method = bytes_mixed_fused(n, nnz, k, cm.reorder)
         kname = "sync-free fused mixed-precision solve"
         # multirow takes its lanes per row from the analysis: lanes is not part of its key
         kl = 0 if cm.kernel == I.KERNEL_MULTIROW else cm.lanes
-        key = (f"fused-mixed[kernel={cm.kernel},lanes={kl},wg={cm.wg},variant={cm.variant},k={k}]"
+        ro = ",reorder=1" if cm.reorder else ""
+        key = (f"fused-mixed[kernel={cm.kernel},lanes={kl},wg={cm.wg},variant={cm.variant}{ro},k={k}]"
                f"@workload{args.workload}")
         traffic = profiled_traffic(key)
     achieved = b_launch / (kmsm * 1e-3) / 1e9

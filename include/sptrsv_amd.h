@@ -310,6 +310,13 @@ typedef struct {
                          keeps 128, every further run of 256 is folded by a
                          wave of its own and added back in a fixed order);
                          -1 walks whole rows */
+    int reorder;      /* ORDER / MULTIROW, FP64 or fused MIXED: 1 solves on a
+                         copy of the matrix with its rows stored in the
+                         analysis's level order (built on first use, kept
+                         with the matrix), so the rows a wave holds are
+                         adjacent in memory; b is read and x written in the
+                         caller's order by the same kernels, and the results
+                         are bit-identical to reorder = 0 */
 } sptrsv_config_t;
 
 /* Relative accuracy the JACOBI path certifies (default 1e-10). */

@@ -62,7 +62,7 @@ class Config(Structure):  # sptrsv_config_t
     _fields_ = [("kernel", c_int), ("lanes", c_int), ("wg", c_int), ("fused", c_int), ("variant", c_int),
                 ("cu_waves", c_int), ("sweeps", c_int), ("outer", c_int), ("window", c_int),
                 ("sweeps_next", c_int), ("inner", c_int), ("claim_rounds", c_int),
-                ("split_rows", c_int)]
+                ("split_rows", c_int), ("reorder", c_int)]
 
 
 class BenchResult(Structure):  # sptrsv_bench_t

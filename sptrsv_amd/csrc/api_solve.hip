@@ -221,6 +221,12 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
     if (out->variant == 3 && out->kernel != SPTRSV_KERNEL_JACOBI &&
         (K > 1 || out->kernel == SPTRSV_KERNEL_MULTIROW))
         return fail(SPTRSV_E_INVALID_ARG, "the in-degree walk (variant 3) takes SIMPLE or ORDER and one right-hand side");
+    if (out->reorder < 0 || out->reorder > 1) return fail(SPTRSV_E_INVALID_ARG, "reorder must be 0 or 1");
+    if (out->reorder == 1 &&
+        ((out->kernel != SPTRSV_KERNEL_ORDER && out->kernel != SPTRSV_KERNEL_MULTIROW) || out->variant == 3 ||
+         precision == SPTRSV_PREC_F32 || (precision == SPTRSV_PREC_MIXED && !out->fused)))
+        return fail(SPTRSV_E_INVALID_ARG, "the level-ordered copy (reorder = 1) takes ORDER or MULTIROW, "
+                                          "variants 0..2, FP64 or fused MIXED");
     return SPTRSV_OK;
 }
 
@@ -397,25 +403,135 @@ int reserve_part(sp_mat_t* m, const dfr_analysis_info_t* info) {
     return SPTRSV_OK;
 }
 
+// ---- level-ordered copy --------------------------------------------------------------
+// Row p of the copy is row iorder[p] of the matrix, its entries in their
+// storage order with every column c renamed inv_iorder[c].  Every dependency
+// of a row sits at an earlier level, hence earlier in iorder, so the copy is
+// lower triangular with the diagonal still last; its own analysis finds the
+// same levels, classes and waves, with iorder the identity (the level sort is
+// stable).  The kernels then fold the same entries in the same order, and the
+// rows of one wave are adjacent in memory.
+__global__ void k_lo_len(const int* __restrict__ ia, const int* __restrict__ iorder, int n, int* __restrict__ len,
+                         int* __restrict__ rows) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int r = iorder[p];
+        len[p] = ia[r + 1] - ia[r];
+        rows[p] = r;
+    }
+}
+
+// One wave per copied row.
+__global__ void k_lo_fill(const int* __restrict__ ia, const int* __restrict__ ja, const double* __restrict__ a,
+                          const float* __restrict__ a32, const int* __restrict__ iorder,
+                          const int* __restrict__ inv, int n, const int* __restrict__ nia, int* __restrict__ nja,
+                          double* __restrict__ na, float* __restrict__ na32) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long waves = (long long)gridDim.x * (blockDim.x / kWave);
+    for (long long p = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave; p < n; p += waves) {
+        const int r = iorder[p];
+        const int s = ia[r], len = ia[r + 1] - s, d = nia[p];
+        for (int q = lane; q < len; q += kWave) {
+            nja[d + q] = inv[ja[s + q]];
+            if (a) na[d + q] = a[s + q];
+            na32[d + q] = a32[s + q];
+        }
+    }
+}
+
+void drop_level_copy(MatPriv* pv) {
+    if (!pv || !pv->lc_mat) return;
+    drop_graphs(pv);  // captured solves hold the copy's arrays
+    free_mat(pv->lc_mat);
+    delete pv->lc_mat;
+    pv->lc_mat = nullptr;
+    if (pv->lc_info) {
+        free_info_arrays(pv->lc_info);
+        free(pv->lc_info);
+        pv->lc_info = nullptr;
+    }
+    if (pv->lc_rows) (void)hipFree(pv->lc_rows);
+    pv->lc_rows = nullptr;
+    pv->lc_src = 0;
+}
+
+// The copy for analysis `info` (built when missing or made for another
+// analysis).  Setup only: allocates, before any capture.
+static int ensure_level_copy(sp_mat_t* m, dfr_analysis_info_t* info) {
+    MatPriv* pv = priv_of(m);
+    const unsigned long long id = analysis_id(info);
+    if (!id || !info->iorder || !info->inv_iorder)
+        return fail(SPTRSV_E_NOT_ANALYSED, "a level-ordered copy needs a level analysis made by this library");
+    if (pv->lc_mat && pv->lc_src == id) return SPTRSV_OK;
+    drop_level_copy(pv);
+    const int n = m->nr;
+    const size_t nz = (size_t)std::max(m->nnz, 1);
+    sp_mat_t* C = new sp_mat_t{};
+    int st = ensure_priv(C);
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && st == SPTRSV_OK) st = hip_fail(e, what, __FILE__, __LINE__);
+    };
+    if (st == SPTRSV_OK) {
+        priv_of(C)->max_row = pv->max_row;
+        C->nr = C->nc = n;
+        C->nnz = m->nnz;
+        chk(hipMalloc(&C->ia, sizeof(int) * ((size_t)n + 1)), "hipMalloc copy ia");
+        chk(hipMalloc(&C->ja, sizeof(int) * nz), "hipMalloc copy ja");
+        chk(hipMalloc(&C->a32, sizeof(float) * nz), "hipMalloc copy a32");
+        if (m->a) chk(hipMalloc(&C->a, sizeof(double) * nz), "hipMalloc copy a");
+        chk(hipMalloc(&pv->lc_rows, sizeof(int) * (size_t)std::max(n, 1)), "hipMalloc row map");
+    }
+    if (st == SPTRSV_OK) chk(hipMemset(C->ia + n, 0, sizeof(int)), "clear ia end");
+    if (st == SPTRSV_OK && n > 0) {
+        hipLaunchKernelGGL(k_lo_len, dim3(stream_grid((size_t)n)), dim3(256), 0, nullptr, m->ia, info->iorder, n,
+                           C->ia, pv->lc_rows);
+        chk(hipGetLastError(), "copy row lengths");
+        if (st == SPTRSV_OK) st = exclusive_scan(C->ia, C->ia, (long long)n + 1, nullptr);
+        if (st == SPTRSV_OK) {
+            const unsigned g = (unsigned)std::min<long long>(65536, ((long long)n + 3) / 4);
+            hipLaunchKernelGGL(k_lo_fill, dim3(g), dim3(256), 0, nullptr, m->ia, m->ja, m->a, m->a32, info->iorder,
+                               info->inv_iorder, n, C->ia, C->ja, C->a, C->a32);
+            chk(hipGetLastError(), "copy entries");
+        }
+    }
+    if (st == SPTRSV_OK) chk(hipDeviceSynchronize(), "level-ordered copy");
+    dfr_analysis_info_t* ci = nullptr;
+    if (st == SPTRSV_OK) st = multirow_analysis_base_GPU(&ci, C, info->mode == ORDER ? ORDER : ALL);
+    if (st == SPTRSV_OK) st = reserve_part(m, ci);  // the copy's segments share the matrix's partials
+    pv->lc_mat = C;
+    pv->lc_info = ci;
+    pv->lc_src = id;
+    if (st != SPTRSV_OK) drop_level_copy(pv);
+    return st;
+}
+
+struct LevelView {
+    const sp_mat_t* mat;
+    const int* row_orig;
+};
+
 // timed = false keeps the pass off the kernel timer (whose event pool may
 // grow, which the allocation-free entry point must not do); part_ext, when
-// given, holds the segment partials instead of the matrix's workspace.
+// given, holds the segment partials instead of the matrix's workspace; lv,
+// when given, names a level-ordered copy whose arrays are walked (with the
+// analysis `info` of the copy) while b and x stay in the caller's order.
 template <class P>
 static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_t& c, int k,
                     const void* b, void* pub, double* x_out, hipStream_t s, bool timed = true,
-                    uint64_t* part_ext = nullptr) {
+                    uint64_t* part_ext = nullptr, const LevelView* lv = nullptr) {
     MatPriv* pv = priv_of(m);
+    const sp_mat_t* sm = lv ? lv->mat : m;
     const int K = pow2_ceil(k);
     SolveArgs A{};
-    A.n = m->nr;
+    A.n = sm->nr;
     A.k = k;
-    A.ia = m->ia;
-    A.ja = m->ja;
-    A.a64 = m->a;
-    A.a32 = m->a32;
+    A.ia = sm->ia;
+    A.ja = sm->ja;
+    A.a64 = sm->a;
+    A.a32 = sm->a32;
     A.b = b;
     A.pub = pub;
     A.x_out = x_out;
+    A.row_orig = lv ? lv->row_orig : nullptr;
     A.iorder = info ? info->iorder : nullptr;
     A.wbase = info ? info->ibase_row64 : nullptr;
     A.wsize = info ? info->ivect_size_wave : nullptr;
@@ -425,7 +541,8 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     A.ticket = pv->counters;
     A.err = pv->counters + 1;
     A.wd_ticks = pv->watchdog_ticks;
-    A.trace = (pv->trace && pv->trace_len >= 3 * (size_t)m->nr * k) ? pv->trace : nullptr;
+    // the trace is kept in the caller's row order, which a copy does not walk
+    A.trace = (!lv && pv->trace && pv->trace_len >= 3 * (size_t)m->nr * k) ? pv->trace : nullptr;
     A.backoff = pv->backoff;
     A.dyn_lds = pv->lds_pad ? pv->lds_pad : lds_pad_for(c.cu_waves, c.wg);
     A.claim_rounds = c.claim_rounds > 0 ? c.claim_rounds : 1;
@@ -462,7 +579,7 @@ static int run_pass(sp_mat_t* m, dfr_analysis_info_t* info, const sptrsv_config_
     size_t words = (size_t)m->nr * k;
     using word = typename P::word;
     hipLaunchKernelGGL((k_prepare<word>), dim3(stream_grid(words)), dim3(256), 0, s,
-                       static_cast<word*>(pub), words, P::kUnsolved, pv->counters, A.part, part_words, m->ia,
+                       static_cast<word*>(pub), words, P::kUnsolved, pv->counters, A.part, part_words, sm->ia,
                        A.indeg, A.indeg ? m->nr : 0);
     SPT_HIP(hipGetLastError());
     int u = c.window > 0 ? c.window : window_for(pv, (kind == KK_MULTIROW || kind == KK_FAST_MULTIROW) ? 32 : c.lanes);
@@ -505,16 +622,27 @@ static int solve_impl(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
                       hipStream_t s) {
     MatPriv* pv = priv_of(m);
     size_t words = (size_t)m->nr * k;
-    if (precision == SPTRSV_PREC_F64)
-        return run_pass<PolF64>(m, info, c, k, b, x, nullptr, s);
-    if (precision == SPTRSV_PREC_F32)
-        return run_pass<PolF32>(m, info, c, k, b, x, nullptr, s);
-    // mixed precision
     const double* bd = static_cast<const double*>(b);
     double* xd = static_cast<double*>(x);
-    if (c.kernel == SPTRSV_KERNEL_JACOBI) return jacobi_solve(m, c, k, bd, xd, s);
     int extra = refine_steps;
-    if (c.fused) {
+    if (c.reorder == 1) {
+        // the level-ordered copy publishes in its own order (the matrix's
+        // workspace) and reads b / writes x in the caller's (FP64 or fused
+        // mixed, checked when resolved)
+        SPT_TRY(ensure_level_copy(m, info));
+        SPT_TRY(reserve(m, k, true, precision == SPTRSV_PREC_MIXED && refine_steps > 1));
+        const LevelView lv{pv->lc_mat, pv->lc_rows};
+        if (precision == SPTRSV_PREC_F64)
+            return run_pass<PolF64>(m, pv->lc_info, c, k, b, pv->pub, xd, s, true, nullptr, &lv);
+        SPT_TRY(run_pass<PolMixed>(m, pv->lc_info, c, k, b, pv->pub, xd, s, true, nullptr, &lv));
+        extra = refine_steps - 1;
+    } else if (precision == SPTRSV_PREC_F64) {
+        return run_pass<PolF64>(m, info, c, k, b, x, nullptr, s);
+    } else if (precision == SPTRSV_PREC_F32) {
+        return run_pass<PolF32>(m, info, c, k, b, x, nullptr, s);
+    } else if (c.kernel == SPTRSV_KERNEL_JACOBI) {
+        return jacobi_solve(m, c, k, bd, xd, s);
+    } else if (c.fused) {
         SPT_TRY(reserve(m, k, true, refine_steps > 1));
         SPT_TRY(run_pass<PolMixed>(m, info, c, k, b, pv->pub, xd, s));
         extra = refine_steps - 1;
@@ -659,10 +787,10 @@ int sptrsv_solve_graph(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, 
     SPT_TRY(resolve_config(mat, info, precision, k, cfg, &c));
     MatPriv* pv = priv_of(mat);
     // the analysis is part of the key: a captured solve holds its arrays
-    const std::array<long long, 20> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
+    const std::array<long long, 21> key{precision, (long long)(uintptr_t)b, (long long)(uintptr_t)x, n, k,
                                         refine_steps, c.kernel, c.lanes, c.wg, c.fused, c.variant, c.cu_waves,
                                         c.sweeps, c.outer, c.window, c.sweeps_next, c.inner, c.claim_rounds,
-                                        c.split_rows, (long long)analysis_id(info)};
+                                        c.split_rows, c.reorder, (long long)analysis_id(info)};
     auto it = pv->graphs.find(key);
     if (it == pv->graphs.end()) {
         // one eager solve sets up every workspace and table (no allocation
@@ -757,6 +885,9 @@ int sptrsv_solve_mixed_noalloc(sp_mat_t* mat, dfr_analysis_info_t* info, const s
     if (c.variant == 3)
         return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve does not run the in-degree walk (its "
                                           "counters are a per-matrix workspace)");
+    if (c.reorder == 1)
+        return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve does not walk a level-ordered copy (the "
+                                          "copy is a per-matrix workspace)");
     // the published words, then (split long rows) the segment partials from
     // the next 256-byte boundary
     const size_t pub_words = (size_t)n * k;
@@ -1028,6 +1159,19 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
             best_t = t;
             bc = c;
         }
+    }
+    // Stage 6 (level-ordered copy): the best ORDER / MULTIROW shape walked on
+    // the copy; the copy is released again when it loses.
+    if (status == SPTRSV_OK && precision != SPTRSV_PREC_F32 && bc.variant != 3 &&
+        (bc.kernel == SPTRSV_KERNEL_ORDER || bc.kernel == SPTRSV_KERNEL_MULTIROW)) {
+        sptrsv_config_t c = bc;
+        c.reorder = 1;
+        const float t = time_cfg(c);
+        if (status == SPTRSV_OK && t < best_t) {
+            best_t = t;
+            bc = c;
+        }
+        if (bc.reorder != 1) drop_level_copy(pv);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

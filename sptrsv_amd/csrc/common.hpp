@@ -63,6 +63,14 @@ struct MatPriv {
     size_t pub_words = 0;
     uint64_t* part = nullptr;    // published partial sums of long-row segments, 2 per segment (device)
     size_t part_words = 0;
+    // level-ordered copy (config reorder = 1): this matrix with its rows in the
+    // level order of an analysis, that copy's own analysis, and the caller's
+    // row of each copied row; built on first use, for the analysis whose id
+    // is lc_src (rebuilt for another), dropped when the values change
+    sp_mat_t* lc_mat = nullptr;
+    dfr_analysis_info_t* lc_info = nullptr;
+    int* lc_rows = nullptr;
+    unsigned long long lc_src = 0;
     // in-degree walk: per-row counters and the off-diagonal pattern by column
     // (built on first use; structure only, so new values keep them)
     int* indeg = nullptr;
@@ -109,7 +117,7 @@ struct MatPriv {
     int jac_members = 1;         // independent systems sharing the certificate (batch)
     // captured solves (sptrsv_solve_graph): key = precision, b, x, n, k,
     // refinement steps and the resolved configuration
-    std::map<std::array<long long, 20>, hipGraphExec_t> graphs;
+    std::map<std::array<long long, 21>, hipGraphExec_t> graphs;
     hipStream_t cap_stream = nullptr;
     bool tuned[3] = {false, false, false};
     sptrsv_config_t tuned_cfg[3] = {};
@@ -158,6 +166,9 @@ int ensure_priv(sp_mat_t* m);
 // changed (waits for the device first: a replay may still be running).
 void drop_graphs(MatPriv* pv);
 void free_mat(sp_mat_t* m);  // device arrays and private state; zeroes *m
+// Releases a matrix's level-ordered copy (if any).
+void drop_level_copy(MatPriv* pv);
+void free_info_arrays(dfr_analysis_info_t* info);
 
 // out[i] = sum_{j<i} in[j] on the device (in and out may alias)
 int exclusive_scan(const int* in, int* out, long long n, hipStream_t s);

@@ -102,6 +102,7 @@ void free_mat(sp_mat_t* m) {
     if (m->a32) (void)hipFree(m->a32);
     MatPriv* pv = priv_of(m);
     if (pv) {
+        drop_level_copy(pv);
         if (pv->counters) (void)hipFree(pv->counters);
         if (pv->pub) (void)hipFree(pv->pub);
         if (pv->part) (void)hipFree(pv->part);
@@ -445,6 +446,7 @@ int sptrsv_update_values(sp_mat_t* mat, const double* vals, long long count, int
     // the margin into their launches, so they are dropped; a tuned Jacobi
     // choice may no longer apply (dominance can be lost)
     if (mat->a) SPT_TRY(jacobi_values_changed(mat));
+    drop_level_copy(pv);  // holds the old values; rebuilt on its next use
     drop_graphs(pv);
     for (int p = 0; p < 3; ++p)
         if (pv->tuned[p] && pv->tuned_cfg[p].kernel == SPTRSV_KERNEL_JACOBI) pv->tuned[p] = false;

@@ -75,7 +75,16 @@ struct SolveArgs {
     int* indeg;
     const int* dep_ptr;     // n + 1
     const int* dep_row;     // rows i > j with an entry in column j, ascending
+    // level-ordered copy: row p of this matrix is the caller's row
+    // row_orig[p], whose b is read and x written there; the published words
+    // stay in this matrix's order (null: the caller's order)
+    const int* row_orig;
 };
+
+// Caller's index of row `row`, right-hand side r.
+__device__ __forceinline__ size_t caller_index(const SolveArgs& A, int row, int r) {
+    return (size_t)(A.row_orig ? A.row_orig[row] : row) * A.k + r;
+}
 
 template <int K>
 __host__ __device__ constexpr int kLog2() {
@@ -199,7 +208,11 @@ struct PolF64 {
     __device__ static word finish(const SolveArgs&, const Diag& d, const Acc& t, Out&) {
         return canon64(t.s * d.piv);
     }
-    __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
+    // x itself is the published array, unless the solve publishes in another
+    // order (a level-ordered copy): then x receives the published word
+    __device__ static void emit(const SolveArgs& A, size_t bi, const Out&, word w) {
+        if (A.x_out) __builtin_nontemporal_store(__longlong_as_double((long long)w), &A.x_out[bi]);
+    }
     __device__ static Out bcast_out(const Out& o, int) { return o; }
     // a segment's partial sum as published words (never all-ones)
     static constexpr int kPartWords = 1;
@@ -231,7 +244,7 @@ struct PolF32 {
     __device__ static word finish(const SolveArgs&, const Diag& d, const Acc& t, Out&) {
         return canon32(t.s * d.piv);
     }
-    __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
+    __device__ static void emit(const SolveArgs&, size_t, const Out&, word) {}
     __device__ static Out bcast_out(const Out& o, int) { return o; }
     static constexpr int kPartWords = 1;
     __device__ static void part_pack(const Acc& a, uint64_t* w) { w[0] = canon32(a.s); }
@@ -280,7 +293,9 @@ struct PolMixed {
                 lane_combine(c.r, s, OpAdd{})};
     }
     struct Out { double x; };
-    __device__ static void emit(const SolveArgs& A, size_t xi, const Out& o) { A.x_out[xi] = o.x; }
+    __device__ static void emit(const SolveArgs& A, size_t bi, const Out& o, word) {
+        __builtin_nontemporal_store(o.x, &A.x_out[bi]);
+    }
     __device__ static Out bcast_out(const Out& o, int src) { return {bcast(o.x, src)}; }
     __device__ static word finish(const SolveArgs&, const Diag& dg, const Acc& t, Out& o) {
         const float piv = dg.piv;
@@ -324,7 +339,7 @@ struct PolLevel {
     __device__ static word finish(const SolveArgs&, const Diag&, const Acc& t, Out&) {
         return (word)(1 + t.m);
     }
-    __device__ static void emit(const SolveArgs&, size_t, const Out&) {}
+    __device__ static void emit(const SolveArgs&, size_t, const Out&, word) {}
     __device__ static Out bcast_out(const Out& o, int) { return o; }
     static constexpr int kPartWords = 1;
     __device__ static void part_pack(const Acc& a, uint64_t* w) { w[0] = (uint64_t)(uint32_t)a.m; }
@@ -384,8 +399,9 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
             }
         }
     }
-    const size_t xi = (size_t)(live ? row : 0) * A.k + r;
-    Acc acc = (live && e == 0) ? P::seed(A, xi) : P::zero();
+    const size_t xi = (size_t)(live ? row : 0) * A.k + r;           // published word
+    const size_t bi = live ? caller_index(A, row, r) : 0;  // b and x (the caller's order)
+    Acc acc = (live && e == 0) ? P::seed(A, bi) : P::zero();
     // The diagonal (and its reciprocal) is independent of the dependencies:
     // load it now so it is off the critical path at publish time.
     typename P::Diag dg{};
@@ -545,12 +561,16 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
             // for with the stores still in flight.
             const int src = __builtin_ctzll(pmask);
             const size_t sxi = pending ? xi : bcast(xi, src);
+            // x's index: the published word's, or (level-ordered copy) the
+            // caller's, kept from the start (a lookup here would wait for
+            // the polls in flight)
+            const size_t sbi = !A.row_orig ? sxi : pending ? bi : bcast(bi, src);
             const word sw = pending ? pend_w : bcast(pend_w, src);
             const typename P::Out so = pending ? pend_o : P::bcast_out(pend_o, src);
             if (pending && row >= lds_lo && row < lds_hi) publish_lds(&s_pub[row - lds_lo], pend_w);
             if (A.trace && pending) A.trace[3 * xi + 2] = wall_clock64();
-            P::emit(A, sxi, so);
             publish_global(&gpub[sxi], sw);
+            P::emit(A, sbi, so, sw);  // x in the caller's order: nobody waits on it
             pending = false;
             asm volatile("" ::: "memory");
             consume();
@@ -804,7 +824,7 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
         }
     }
     const size_t xi = live ? (size_t)row : 0;
-    Acc acc = (live && e == 0) ? P::seed(A, xi) : P::zero();
+    Acc acc = (live && e == 0) ? P::seed(A, caller_index(A, row, 0)) : P::zero();
     typename P::Diag dg{};
     if (live && e == 0) dg = P::diag(A, dpos);
     if (A.trace && live && e == 0) A.trace[3 * xi] = wall_clock64();
@@ -846,9 +866,9 @@ __device__ __forceinline__ void fast_rows(const SolveArgs& A, int L, int lane, i
         if (A.trace) A.trace[3 * xi + 1] = wall_clock64();
         typename P::Out o{};
         const word w = P::finish(A, dg, t, o);
-        P::emit(A, xi, o);
         if (A.trace) A.trace[3 * xi + 2] = wall_clock64();
         publish_global(&gpub[xi], w);
+        P::emit(A, caller_index(A, row, 0), o, w);
     }
 }
 
@@ -958,9 +978,9 @@ __device__ __forceinline__ void indeg_row(const SolveArgs& A, int L, int lane, i
         if (A.trace) A.trace[3 * xi + 1] = wall_clock64();
         typename P::Out o{};
         const word w = P::finish(A, dg, t, o);
-        P::emit(A, xi, o);
         if (A.trace) A.trace[3 * xi + 2] = wall_clock64();
         publish_global(&gpub[xi], w);
+        P::emit(A, xi, o, w);
     }
     // hand the row to its dependents
     for (int q = A.dep_ptr[row] + lane; q < A.dep_ptr[row + 1]; q += kWave)

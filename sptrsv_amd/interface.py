@@ -275,9 +275,9 @@ def csr_L_solve_mixed(mat, info, b: DeviceArray, x: DeviceArray, k: int = 1, ref
 
 
 def _cfg(kernel=KERNEL_AUTO, lanes=0, wg=0, fused=1, variant=0, cu_waves=0, sweeps=0, outer=0,
-         window=0, sweeps_next=0, inner=0, claim_rounds=0, split_rows=0) -> Config:
+         window=0, sweeps_next=0, inner=0, claim_rounds=0, split_rows=0, reorder=0) -> Config:
     return Config(kernel, lanes, wg, fused, variant, cu_waves, sweeps, outer, window, sweeps_next, inner,
-                  claim_rounds, split_rows)
+                  claim_rounds, split_rows, reorder)
 
 
 def solve(mat, info, precision: int, b: DeviceArray, x: DeviceArray, k: int = 1, refine_steps: int = 1,
