@@ -166,6 +166,7 @@ static int resolve_config(sp_mat_t* m, dfr_analysis_info_t* info, int precision,
                 (pv->tuned_cfg[precision].kernel == SPTRSV_KERNEL_JACOBI && precision == SPTRSV_PREC_MIXED))) {
         *out = pv->tuned_cfg[precision];
         if (cfg) out->fused = cfg->fused;
+        if (!out->fused) out->reorder = 0;  // the copy serves the fused pass only
     } else {
         default_config(m, info, precision, k, out);
         if (cfg) out->fused = cfg->fused;

@@ -20,8 +20,9 @@
 // Certificate: with delta = min_i(|L_ii| - sum_j |L_ij|) > 0, |L^-1|_inf <=
 // 1/delta, so |x - x*|_inf <= |b - L x|_inf / delta.  A final residual pass
 // takes max |r| and max |x| per independent system (one, or each member of a
-// batch) and k_jac_certify compares the bound with tol * |x|_inf, leaving the
-// verdict in a device word (no host round trip).
+// batch, and each right-hand-side column) and k_jac_certify compares the
+// bound with tol * |x|_inf, leaving the verdict in a device word (no host
+// round trip) and clearing the norms for the next certificate.
 //
 // Kernel layout (both passes): rows are cut into tiles by nnz windows -- tile
 // t holds the rows whose first entry lies in [t*W, (t+1)*W), W = 1024/K
@@ -702,21 +703,18 @@ __global__ void k_jac_tiles(const int* ia, int n, int window, int ntiles, int4* 
     tiles[t] = make_int4(r0, r1, ia[r0], ia[r1]);
 }
 
-__global__ void k_jac_zero_norms(unsigned long long* norms, size_t words) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
-        norms[i] = 0ull;
-}
-
 // status = 1 unless, for every slot (member and right-hand-side column),
-// |r|_inf / delta <= tol * |x|_inf.
-__global__ void k_jac_certify(const unsigned long long* norms, int members, double delta, double tol,
-                              int* status) {
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < members; m += gridDim.x * blockDim.x) {
+// |r|_inf / delta <= tol * |x|_inf.  The norm words are cleared as they are
+// read, ready for the next certificate (they are zero from allocation on).
+__global__ void k_jac_certify(unsigned long long* norms, int slots, double delta, double tol, int* status) {
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < slots; m += gridDim.x * blockDim.x) {
         unsigned long long rb = 0, xb = 0;
         for (int rep = 0; rep < kNormReps; ++rep) {
-            const unsigned long long* s = norms + ((size_t)m * kNormReps + rep) * kNormStride;
-            rb = max(rb, s[0]);
-            xb = max(xb, s[1]);
+            unsigned long long* w = norms + ((size_t)m * kNormReps + rep) * kNormStride;
+            rb = max(rb, w[0]);
+            xb = max(xb, w[1]);
+            w[0] = 0ull;
+            w[1] = 0ull;
         }
         const double rn = __longlong_as_double((long long)rb);
         const double xn = __longlong_as_double((long long)xb);
@@ -817,6 +815,7 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
         SPT_HIP(hipMalloc(&pv->jac_b, words * sizeof(float)));
         pv->jac_len = words;
     }
+    // the norm words: zero between certificates (the certificate clears them)
     const size_t nw = (size_t)std::max(1, pv->jac_members) * k * kNormReps * kNormStride;
     if (pv->jac_norms_len < nw) {
         drop_graphs(pv);
@@ -824,6 +823,8 @@ static int ensure_jacobi_ws(sp_mat_t* m, int k) {
         pv->jac_norms = nullptr;
         pv->jac_norms_len = 0;
         SPT_HIP(hipMalloc(&pv->jac_norms, nw * sizeof(unsigned long long)));
+        SPT_HIP(hipMemset(pv->jac_norms, 0, nw * sizeof(unsigned long long)));
+        SPT_HIP(hipDeviceSynchronize());
         pv->jac_norms_len = nw;
     }
     if (!pv->jac_sval) {
@@ -1018,13 +1019,12 @@ static int jacobi_schedule(sp_mat_t* m, const sptrsv_config_t& c, int k, const d
         if (ev) SPT_HIP(hipEventRecord(ev[1], s));
     }
     // certificate: final FP64 residual with max-norms per member and column
+    // (the norms are zero: cleared by the previous certificate)
     const int slots = with.members * k;
-    const size_t nw = (size_t)slots * kNormReps * kNormStride;
-    hipLaunchKernelGGL(k_jac_zero_norms, dim3(ceil_div((long long)nw, 256)), dim3(256), 0, s, pv->jac_norms, nw);
     hipLaunchKernelGGL((k_jac_resid<K>), dim3(tiles), dim3(kTileThreads), 0, s, A, b, (const double*)x,
                        (float*)nullptr, with, epl);
-    hipLaunchKernelGGL(k_jac_certify, dim3(ceil_div(slots, 256)), dim3(256), 0, s, pv->jac_norms,
-                       slots, pv->margin, pv->tol, pv->counters + 3);
+    hipLaunchKernelGGL(k_jac_certify, dim3(ceil_div(slots, 256)), dim3(256), 0, s, pv->jac_norms, slots,
+                       pv->margin, pv->tol, pv->counters + 3);
     SPT_HIP(hipGetLastError());
     return SPTRSV_OK;
 }

@@ -184,3 +184,20 @@ def test_copy_refusals():
     for a in (db, dx, d32, x32):
         a.free()
     I.free_device_analysis(mat, info)
+
+
+def test_tuned_copy_serves_auto_solves():
+    print("level-ordered copy: autotune, then automatic solves (fused and unfused)", flush=True)
+    s = I.generate(3, 200000)
+    mat = I.allocate_memory(s.ia, s.ja, s.a)
+    info = I.multirow_analysis_base_GPU(mat, I.ALL)
+    ref = O.fwd_f64(s.ia, s.ja, s.a, s.b)
+    for prec, tol in ((I.PREC_F64, 1e-12), (I.PREC_MIXED, 1e-10)):
+        cfg, ms = I.autotune(mat, info, prec, reps=2)
+        print(f"  tuned precision {prec}: kernel {cfg.kernel} variant {cfg.variant} reorder {cfg.reorder}", flush=True)
+        assert ms > 0 and cfg.reorder in (0, 1)
+        fused = (1, 0) if prec == I.PREC_MIXED else (1,)
+        for f in fused:                          # an unfused call drops the copy, never fails on it
+            x = solve(mat, info, prec, s.b, kernel=I.KERNEL_AUTO, fused=f)
+            assert relerr(x, ref) <= tol, (prec, f)
+    I.free_device_analysis(mat, info)
