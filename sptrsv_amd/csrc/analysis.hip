@@ -707,6 +707,61 @@ static int structure_hash(sp_mat_t* m, unsigned long long* h) {
 
 static bool has_warps(const dfr_analysis_info_t* info) { return info->mode == MULTIROW || info->mode == ALL; }
 
+// Levels of a level-ordered copy (row p is row iorder[p]) and its identity
+// order.
+__global__ void k_copy_levels(const int* __restrict__ level, const int* __restrict__ iorder, int n,
+                              int* __restrict__ out_level, int* __restrict__ out_iorder, int* __restrict__ out_inv) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        out_level[p] = level[iorder[p]];
+        out_iorder[p] = p;
+        out_inv[p] = p;
+    }
+}
+
+// The analysis of `copy`, the level-ordered copy of the matrix `src` was made
+// for, derived from `src` instead of recomputed: the copy's levels are src's
+// permuted, its iorder is the identity (src's order is a stable sort by key,
+// which the copy's rows already follow), and its buckets and warp arrays,
+// which index positions in iorder, are src's.  Only the segment table, which
+// names entry positions, is built for the copy.
+int level_copy_analysis(const dfr_analysis_info_t* src, sp_mat_t* copy, dfr_analysis_info_t* dst) {
+    if (!src || !owns_info(src) || !src->iorder || !src->level || !dst || !copy || src->n != copy->nr)
+        return fail(SPTRSV_E_NOT_ANALYSED, "level-ordered copy: no level analysis to derive from");
+    const bool warps = has_warps(src);
+    dfr_analysis_info_t t = *src;
+    for (const InfoArray& a : info_arrays(&t, warps)) *a.dev = nullptr;
+    t.row_ctr = nullptr;
+    t.priv = nullptr;
+    free_info_arrays(dst);
+    *dst = t;
+    dst->priv = new_info_priv();
+    int st = SPTRSV_OK;
+    if (hipMalloc(&dst->row_ctr, sizeof(int)) != hipSuccess || hipMemset(dst->row_ctr, 0, sizeof(int)) != hipSuccess)
+        st = fail(SPTRSV_E_ALLOC, "row_ctr");
+    const std::vector<InfoArray> from = info_arrays(const_cast<dfr_analysis_info_t*>(src), warps),
+                                 to = info_arrays(dst, warps);
+    for (size_t i = 0; i < to.size() && st == SPTRSV_OK; ++i) {
+        if (to[i].len == 0) continue;
+        const size_t bytes = sizeof(int) * (size_t)to[i].len;
+        if (hipMalloc(to[i].dev, bytes) != hipSuccess) {
+            st = fail(SPTRSV_E_ALLOC, "analysis arrays");
+            break;
+        }
+        const bool derived = to[i].dev == &dst->level || to[i].dev == &dst->iorder || to[i].dev == &dst->inv_iorder;
+        if (!derived && hipMemcpy(*to[i].dev, *from[i].dev, bytes, hipMemcpyDeviceToDevice) != hipSuccess)
+            st = fail(SPTRSV_E_HIP, "analysis arrays");
+    }
+    if (st == SPTRSV_OK && dst->n > 0) {
+        hipLaunchKernelGGL(k_copy_levels, dim3(grid_for(dst->n)), dim3(256), 0, nullptr, src->level, src->iorder,
+                           dst->n, dst->level, dst->iorder, dst->inv_iorder);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            st = fail(SPTRSV_E_HIP, "copy levels");
+    }
+    if (st == SPTRSV_OK) st = build_segments(dst, copy);
+    if (st != SPTRSV_OK) free_info_arrays(dst);
+    return st;
+}
+
 }  // namespace sptrsv
 
 using namespace sptrsv;

@@ -408,10 +408,11 @@ int reserve_part(sp_mat_t* m, const dfr_analysis_info_t* info) {
 // Row p of the copy is row iorder[p] of the matrix, its entries in their
 // storage order with every column c renamed inv_iorder[c].  Every dependency
 // of a row sits at an earlier level, hence earlier in iorder, so the copy is
-// lower triangular with the diagonal still last; its own analysis finds the
-// same levels, classes and waves, with iorder the identity (the level sort is
-// stable).  The kernels then fold the same entries in the same order, and the
-// rows of one wave are adjacent in memory.
+// lower triangular with the diagonal still last; its analysis has the same
+// levels, classes and waves, with iorder the identity (the level sort is
+// stable), and is derived from the matrix's rather than recomputed.  The
+// kernels then fold the same entries in the same order, and the rows of one
+// wave are adjacent in memory.
 __global__ void k_lo_len(const int* __restrict__ ia, const int* __restrict__ iorder, int n, int* __restrict__ len,
                          int* __restrict__ rows) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
@@ -495,8 +496,9 @@ static int ensure_level_copy(sp_mat_t* m, dfr_analysis_info_t* info) {
         }
     }
     if (st == SPTRSV_OK) chk(hipDeviceSynchronize(), "level-ordered copy");
-    dfr_analysis_info_t* ci = nullptr;
-    if (st == SPTRSV_OK) st = multirow_analysis_base_GPU(&ci, C, info->mode == ORDER ? ORDER : ALL);
+    dfr_analysis_info_t* ci = static_cast<dfr_analysis_info_t*>(calloc(1, sizeof(dfr_analysis_info_t)));
+    if (!ci && st == SPTRSV_OK) st = fail(SPTRSV_E_ALLOC, "copy analysis struct");
+    if (st == SPTRSV_OK) st = level_copy_analysis(info, C, ci);
     if (st == SPTRSV_OK) st = reserve_part(m, ci);  // the copy's segments share the matrix's partials
     pv->lc_mat = C;
     pv->lc_info = ci;

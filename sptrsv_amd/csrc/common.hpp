@@ -169,6 +169,9 @@ void free_mat(sp_mat_t* m);  // device arrays and private state; zeroes *m
 // Releases a matrix's level-ordered copy (if any).
 void drop_level_copy(MatPriv* pv);
 void free_info_arrays(dfr_analysis_info_t* info);
+// The analysis of `copy`, the level-ordered copy made for analysis `src`,
+// derived from `src` (levels permuted, iorder the identity, the rest shared).
+int level_copy_analysis(const dfr_analysis_info_t* src, sp_mat_t* copy, dfr_analysis_info_t* dst);
 
 // out[i] = sum_{j<i} in[j] on the device (in and out may alias)
 int exclusive_scan(const int* in, int* out, long long n, hipStream_t s);
