@@ -10,7 +10,9 @@ workloads.
 Bars: analysis arrays bit-exact; FP64 within 1e-12 relative (max-norm) of
 the oracle's serial FP64 forward substitution; mixed precision (FP32 bulk,
 FP64 refinement) within 1e-10, with the Jacobi certificate accepted where
-that path runs; batch members each on their own.  The reference's own check
+that path runs; batch members each on their own; the tuned persistent
+shapes bit-identical with and without the level-ordered copy.  The
+reference's own check
 is exact x == 1 on the user's full matrix (test.cu:13-16,
 matrix_properties.h:7-29); the known-answer inputs are covered in
 test_gpu_solve.py.
@@ -93,17 +95,27 @@ def test_powerlaw_2e6():
     w = Full(3, 2000000)
     nl = w.check_analysis()
     say(f"analysis bit-exact, {nl} levels")
+    tuned = dict(kernel=I.KERNEL_MULTIROW, lanes=32, wg=1024, variant=2, cu_waves=8)
     cases = {
         "f64 multirow": (I.PREC_F64, dict(kernel=I.KERNEL_MULTIROW, wg=256, window=16)),
         "f64 simple lean": (I.PREC_F64, dict(kernel=I.KERNEL_SIMPLE, lanes=32, wg=256, variant=1, cu_waves=4)),
         "mixed fused multirow": (I.PREC_MIXED, dict(kernel=I.KERNEL_MULTIROW, wg=256, window=16)),
+        "f64 multirow persistent": (I.PREC_F64, tuned),
+        "f64 multirow persistent, level-ordered copy": (I.PREC_F64, dict(tuned, reorder=1)),
+        "mixed multirow persistent": (I.PREC_MIXED, tuned),
+        "mixed multirow persistent, level-ordered copy": (I.PREC_MIXED, dict(tuned, reorder=1)),
     }
+    got = {}
     for name, (prec, cfg) in cases.items():
         st, x = w.solve(prec, **cfg)
         err = relerr(x, w.ref)
         say(f"{name}: status {st}, rel err {err:.2e}")
         assert st == 0, (name, L.last_error())
         assert err <= (1e-12 if prec == I.PREC_F64 else 1e-10), (name, err)
+        got[name] = x
+    for p in ("f64", "mixed"):   # the copy changes no bit
+        a, c = got[f"{p} multirow persistent"], got[f"{p} multirow persistent, level-ordered copy"]
+        assert np.array_equal(a.view(np.uint64), c.view(np.uint64)), p
     w.close()
 
 
@@ -120,7 +132,12 @@ def test_laplacian3d_128_16rhs():
         "mixed fused order persistent": (I.PREC_MIXED, dict(kernel=I.KERNEL_ORDER, lanes=2, wg=256, variant=2,
                                                             cu_waves=8)),
         "mixed jacobi": (I.PREC_MIXED, dict(kernel=I.KERNEL_JACOBI, lanes=1, wg=256, sweeps=30, outer=3)),
+        "mixed fused order persistent 1024": (I.PREC_MIXED, dict(kernel=I.KERNEL_ORDER, lanes=1, wg=1024,
+                                                                 variant=2, cu_waves=12)),
+        "mixed fused order persistent 1024, level-ordered copy": (
+            I.PREC_MIXED, dict(kernel=I.KERNEL_ORDER, lanes=1, wg=1024, variant=2, cu_waves=12, reorder=1)),
     }
+    got = {}
     for name, (prec, cfg) in cases.items():
         st, x = w.solve(prec, **cfg)
         err = relerr(x, w.ref)
@@ -131,6 +148,9 @@ def test_laplacian3d_128_16rhs():
         X, R = x.reshape(w.s.n, 16), w.ref.reshape(w.s.n, 16)
         worst = max(relerr(X[:, c], R[:, c]) for c in range(16))
         assert worst <= (1e-12 if prec == I.PREC_F64 else 1e-10), (name, worst)
+        got[name] = x
+    a, c = got["mixed fused order persistent 1024"], got["mixed fused order persistent 1024, level-ordered copy"]
+    assert np.array_equal(a.view(np.uint64), c.view(np.uint64))   # the copy changes no bit
     w.close()
 
 
