@@ -216,6 +216,10 @@ struct PolF64 {
     __device__ static Out bcast_out(const Out& o, int) { return o; }
     // a segment's partial sum as published words (never all-ones)
     static constexpr int kPartWords = 1;
+    // general engine: form every poll address of a turn before the first
+    // poll goes out (FP64 only: measured faster there, slower for the
+    // fused mixed pass)
+    static constexpr bool kPollsTogether = true;
     __device__ static void part_pack(const Acc& a, uint64_t* w) { w[0] = canon64(a.s); }
     __device__ static Acc part_unpack(const uint64_t* w) { return {__longlong_as_double((long long)w[0])}; }
 };
@@ -247,6 +251,7 @@ struct PolF32 {
     __device__ static void emit(const SolveArgs&, size_t, const Out&, word) {}
     __device__ static Out bcast_out(const Out& o, int) { return o; }
     static constexpr int kPartWords = 1;
+    static constexpr bool kPollsTogether = false;
     __device__ static void part_pack(const Acc& a, uint64_t* w) { w[0] = canon32(a.s); }
     __device__ static Acc part_unpack(const uint64_t* w) { return {__uint_as_float((uint32_t)w[0])}; }
 };
@@ -308,6 +313,7 @@ struct PolMixed {
         return ((uint64_t)hi << 32) | lo;  // never all-ones: lo is canonical
     }
     static constexpr int kPartWords = 2;
+    static constexpr bool kPollsTogether = false;
     __device__ static void part_pack(const Acc& a, uint64_t* w) {
         w[0] = ((uint64_t)canon32(a.d) << 32) | canon32(a.x0);
         w[1] = canon64(a.r);
@@ -342,6 +348,7 @@ struct PolLevel {
     __device__ static void emit(const SolveArgs&, size_t, const Out&, word) {}
     __device__ static Out bcast_out(const Out& o, int) { return o; }
     static constexpr int kPartWords = 1;
+    static constexpr bool kPollsTogether = false;
     __device__ static void part_pack(const Acc& a, uint64_t* w) { w[0] = (uint64_t)(uint32_t)a.m; }
     __device__ static Acc part_unpack(const uint64_t* w) { return {(int)(uint32_t)w[0]}; }
 };
@@ -464,17 +471,46 @@ __device__ __forceinline__ void sf_rows(const SolveArgs& A, int L, int lane, int
         // (1) global polls of the window entries not folded yet
         word w[U];
         bool in_lds[U];
+        if constexpr (P::kPollsTogether) {
+            // every address formed before the first poll goes out (a column
+            // still arriving from the previous turn's window load is waited
+            // for here), so the polls leave back to back
+            bool in_mem[U];
+            const word* ad[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = p + u * L;
-            w[u] = P::kUnsolved;
-            in_lds[u] = false;
-            if (!edone && u >= kc && q < eend) {
-                const int c = col[u];
-                if (c >= lds_lo && c < lds_hi)
-                    in_lds[u] = true;
-                else
-                    w[u] = poll_global(&gpub[(size_t)c * A.k + r]);
+            for (int u = 0; u < U; ++u) {
+                const int q = p + u * L;
+                in_lds[u] = in_mem[u] = false;
+                ad[u] = gpub;
+                if (!edone && u >= kc && q < eend) {
+                    const int c = col[u];
+                    if (c >= lds_lo && c < lds_hi) {
+                        in_lds[u] = true;
+                    } else {
+                        in_mem[u] = true;
+                        ad[u] = &gpub[(size_t)c * A.k + r];
+                    }
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                w[u] = P::kUnsolved;
+                if (in_mem[u]) w[u] = poll_global(ad[u]);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = p + u * L;
+                w[u] = P::kUnsolved;
+                in_lds[u] = false;
+                if (!edone && u >= kc && q < eend) {
+                    const int c = col[u];
+                    if (c >= lds_lo && c < lds_hi)
+                        in_lds[u] = true;
+                    else
+                        w[u] = poll_global(&gpub[(size_t)c * A.k + r]);
+                }
             }
         }
         // the next partial of a split row whose entries are folded

@@ -141,11 +141,26 @@ def test_copy_follows_new_values_and_analyses():
     shape = dict(kernel=I.KERNEL_MULTIROW, lanes=32, wg=256, variant=2)
     x = same_as_original(mat, info, I.PREC_F64, t.b, **shape)
     assert relerr(x, O.fwd_f64(t.ia, t.ja, t.a, t.b)) <= 1e-12
+    # a captured solve on the copy, then new values: the capture (which holds
+    # the copy's arrays) is dropped with the copy and made again
+    db = I.DeviceArray.from_numpy(t.b)
+    dx = I.DeviceArray(t.n, np.float64)
+    gcfg = I._cfg(**shape, reorder=1)
+    I.solve_graph(mat, info, I.PREC_MIXED, db, dx, gcfg)
+    I.synchronize()
+    assert relerr(dx.numpy(), O.fwd_f64(t.ia, t.ja, t.a, t.b)) <= 1e-10
     a2 = t.a.copy()
     off = np.ones(len(a2), bool)
     off[t.ia[1:] - 1] = False
     a2[off] *= -0.5
     I.update_values(mat, a2)
+    dx.fill_bytes(0)
+    I.solve_graph(mat, info, I.PREC_MIXED, db, dx, gcfg)
+    I.synchronize()
+    assert I.check_watchdog(mat) == 0
+    assert relerr(dx.numpy(), O.fwd_f64(t.ia, t.ja, a2, t.b)) <= 1e-10
+    db.free()
+    dx.free()
     x2 = same_as_original(mat, info, I.PREC_F64, t.b, **shape)
     assert relerr(x2, O.fwd_f64(t.ia, t.ja, a2, t.b)) <= 1e-12
     # an ORDER analysis of the same matrix: the copy is made again for it
