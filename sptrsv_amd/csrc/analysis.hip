@@ -94,6 +94,48 @@ __global__ void k_keys(int n, const int* __restrict__ ia, const int* __restrict_
     }
 }
 
+// The same keys with the bucket histogram kept in LDS per workgroup (when
+// the 7 * nlevs buckets fit), each workgroup taking one contiguous range of
+// rows: a level holds up to hundreds of thousands of rows, and one global
+// atomic per row on its bucket serialises on those few hot words.  Each
+// workgroup adds its nonzero counts once; level sizes follow from the
+// buckets (k_lev_sizes).
+constexpr int kKeyLdsBins = 12288;
+__global__ void k_keys_lds(int n, const int* __restrict__ ia, const int* __restrict__ lev, int nlevs,
+                           int* __restrict__ key, int* __restrict__ vals, int* bucket_cnt, int* bad) {
+    extern __shared__ int s_bins[];
+    const int nb = 7 * nlevs;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s_bins[b] = 0;
+    __syncthreads();
+    const long long chunk = ((long long)n + gridDim.x - 1) / gridDim.x;
+    const long long r0 = (long long)blockIdx.x * chunk;
+    const long long r1 = r0 + chunk < n ? r0 + chunk : n;
+    for (long long i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        int l = lev[i];
+        if (l < 1 || l > nlevs) {
+            atomicOr(bad, 1);
+            l = 1;
+        }
+        const int kk = 7 * (l - 1) + row_class(ia[i + 1] - ia[i] - 1);
+        key[i] = kk;
+        vals[i] = (int)i;
+        atomicAdd(&s_bins[kk], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const int c = s_bins[b];
+        if (c) atomicAdd(&bucket_cnt[b], c);
+    }
+}
+
+__global__ void k_lev_sizes(const int* __restrict__ bucket_cnt, int nlevs, int* __restrict__ lev_size) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < nlevs; l += gridDim.x * blockDim.x) {
+        int t = 0;
+        for (int c = 0; c < 7; ++c) t += bucket_cnt[7 * l + c];
+        lev_size[l] = t;
+    }
+}
+
 // ---- exclusive scan (int), any length ------------------------------------------------
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
@@ -617,8 +659,15 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
     SPT_HIP(hipMalloc(&info->lev_size, sizeof(int) * (size_t)nlevs));
     hipLaunchKernelGGL(k_fill_int, dim3(grid_for(nb + 1)), dim3(256), 0, s, bucket_cnt, (size_t)(nb + 1), 0);
     hipLaunchKernelGGL(k_fill_int, dim3(grid_for(nlevs)), dim3(256), 0, s, info->lev_size, (size_t)nlevs, 0);
-    hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(256), 0, s, n, m->ia, info->level, nlevs, key,
-                       info->iorder, bucket_cnt, info->lev_size, pv->counters + 2);
+    if (nb <= kKeyLdsBins) {
+        const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>(((long long)n + 2047) / 2048, 1024));
+        hipLaunchKernelGGL(k_keys_lds, dim3(g), dim3(256), sizeof(int) * (size_t)nb, s, n, m->ia, info->level, nlevs,
+                           key, info->iorder, bucket_cnt, pv->counters + 2);
+        hipLaunchKernelGGL(k_lev_sizes, dim3(grid_for(nlevs)), dim3(256), 0, s, bucket_cnt, nlevs, info->lev_size);
+    } else {
+        hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(256), 0, s, n, m->ia, info->level, nlevs, key,
+                           info->iorder, bucket_cnt, info->lev_size, pv->counters + 2);
+    }
     SPT_HIP(hipGetLastError());
     {
         int bad = 0;
