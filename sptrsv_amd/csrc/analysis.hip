@@ -589,8 +589,10 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
     // kernel is chosen for speed.  Rows of moderate length (mean >= 6, none
     // over 256 entries) take the lean one-row-per-wave pass with its lanes
     // covering a typical row and 4 resident waves per CU, which keeps the
-    // dependency hand-off short; other matrices keep the reference's tile
-    // rule on the general engine.
+    // dependency hand-off short.  Short rows (mean < 6) take one lane per
+    // row on the general engine: 64 consecutive rows per wave keep many more
+    // rows in flight, which a natural-order walk of a grid-like matrix needs
+    // to find ready rows.  Other matrices keep the reference's tile rule.
     int kind = KK_SIMPLE;
     unsigned dyn_lds = 0;
     const int avg = n > 0 ? m->nnz / n : 0;
@@ -600,6 +602,8 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
         while (lanes < avg && lanes < kWave) lanes <<= 1;
         wg = 256;
         dyn_lds = lds_pad_for(4, wg);
+    } else if (avg < 6) {
+        lanes = 1;
     }
     {
         SolveArgs A{};
