@@ -7,7 +7,8 @@
 // with the ticket counter; tasks of all matrices are interleaved block by
 // block, so every matrix starts its critical path immediately, and each
 // matrix runs with its own lanes-per-row choice (heuristic, or timed per
-// matrix by sptrsv_batch_tune).  The prepare kernel and the solve are
+// matrix by sptrsv_batch_tune and then capped for the whole launch, which is
+// bound by throughput rather than by any one member's chain).  The prepare kernel and the solve are
 // captured once in a hipGraph and replayed.
 //
 // Mixed precision adds a second per-matrix choice: a strictly diagonally
@@ -455,16 +456,17 @@ int build_sub(sptrsv_batch_t* bt) {
 }
 
 // Mean time of `reps` whole batched MIXED solves (after one warm-up).
-float time_batch(sptrsv_batch_t* bt, const double* b, double* x, int reps, int* st) {
+float time_batch(sptrsv_batch_t* bt, const double* b, double* x, int reps, int* st,
+                 int precision = SPTRSV_PREC_MIXED) {
     hipEvent_t e0, e1;
     *st = SPTRSV_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         *st = fail(SPTRSV_E_HIP, "event create");
         return 1e30f;
     }
-    *st = batch_solve_all(bt, SPTRSV_PREC_MIXED, b, x, nullptr);
+    *st = batch_solve_all(bt, precision, b, x, nullptr);
     (void)hipEventRecord(e0, nullptr);
-    for (int r = 0; r < reps && *st == SPTRSV_OK; ++r) *st = batch_solve_all(bt, SPTRSV_PREC_MIXED, b, x, nullptr);
+    for (int r = 0; r < reps && *st == SPTRSV_OK; ++r) *st = batch_solve_all(bt, precision, b, x, nullptr);
     (void)hipEventRecord(e1, nullptr);
     (void)hipEventSynchronize(e1);
     float ms = 0;
@@ -658,6 +660,42 @@ int apply_choice(sptrsv_batch_t* bt, const std::vector<int>& jac, float* jac_ms)
     return build_tasks(bt);
 }
 
+// Lanes per row of the sync-free members launched together.  Each member's
+// lanes were timed with the member alone, where its dependency chain bounds
+// it; launched together the members are bound by throughput, where fewer
+// lanes per row (more rows per wave, every lane busy) can win: the 128
+// random members of the 256-system benchmark (about 7 entries per row) take
+// 2.9 ms at their own 8 lanes and 1.7 ms at 1 or 2.  The whole batch is
+// timed with the members' own lanes and with every sync-free member capped
+// at 2 and at 1 lane per row, and the fastest is kept.
+int tune_concurrent_lanes(sptrsv_batch_t* bt, TuneScratch& ts, int precision) {
+    const std::vector<int> own = bt->lanes;
+    bt->use_sub = false;  // the task launch is what the lanes shape
+    SPT_TRY(build_tasks(bt));
+    int st = SPTRSV_OK;
+    float best = time_batch(bt, ts.b, ts.x, 3, &st, precision);
+    SPT_TRY(st);
+    std::vector<int> best_lanes = own;
+    for (int cap : {2, 1}) {
+        bool changed = false;
+        for (size_t m = 0; m < own.size(); ++m) {
+            const bool jacobi = precision == SPTRSV_PREC_MIXED && bt->choice[m] == SPTRSV_KERNEL_JACOBI;
+            bt->lanes[m] = jacobi ? own[m] : std::min(own[m], cap);
+            changed |= bt->lanes[m] != own[m];
+        }
+        if (!changed) continue;
+        SPT_TRY(build_tasks(bt));
+        const float t = time_batch(bt, ts.b, ts.x, 3, &st, precision);
+        SPT_TRY(st);
+        if (t < best) {
+            best = t;
+            best_lanes = bt->lanes;
+        }
+    }
+    bt->lanes = best_lanes;
+    return build_tasks(bt);
+}
+
 // MIXED per-matrix choice between the sync-free launch and Jacobi.  The
 // Jacobi time J of all diagonally dominant matrices is measured first; a
 // dominant matrix whose sync-free time alone exceeds J / 2 is "slow".  Then
@@ -741,6 +779,7 @@ int tune_mixed(sptrsv_batch_t* bt, TuneScratch& ts) {
         }
     }
     SPT_TRY(apply_choice(bt, best_set, nullptr));
+    SPT_TRY(tune_concurrent_lanes(bt, ts, SPTRSV_PREC_MIXED));
     // The sync-free matrices either through the ticketed task launch or as a
     // matrix of their own (level analysis, tuned kernel); the two halves
     // concurrently on two streams or one after the other (the concurrent
@@ -819,6 +858,10 @@ int sptrsv_batch_tune(sptrsv_batch_t* bt, int precision, int tune) {
         bt->jac_cfg = jc;
         bt->serial = true;
         bt->use_sub = false;
+        // launched together the sync-free members are bound by throughput:
+        // rows of under 12 entries take at most 2 lanes (more rows per wave)
+        for (int m = 0; m < nmat; ++m)
+            if (bt->choice[m] != SPTRSV_KERNEL_JACOBI && bt->avg[m] < 12) bt->lanes[m] = std::min(bt->lanes[m], 2);
         free_sub(bt);
         SPT_TRY(install_jacobi(bt));
         return build_tasks(bt);
@@ -845,7 +888,7 @@ int sptrsv_batch_tune(sptrsv_batch_t* bt, int precision, int tune) {
         bt->t_sync[m] = best;
     }
     SPT_TRY(sptrsv_check(&bt->mat, nullptr));
-    return build_tasks(bt);
+    return tune_concurrent_lanes(bt, ts, precision);
 }
 
 int sptrsv_batch_solve(sptrsv_batch_t* bt, int precision, const double* b, double* x, hipStream_t stream) {
