@@ -1134,11 +1134,15 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
     // (variant 2: exact solves of the tiles' diagonal blocks)
     const std::vector<int> point_sweeps{4, 6, 8, 10, 12, 14, 16, 20, 24};
     const std::vector<int> block_sweeps{2, 3, 4, 5, 6, 8, 10, 12, 16};
-    for (int var : {0, 2}) {
-    if (var == 2 && k != 1) break;
+    // (block sweeps with 4 and with 8 in-tile iterations: more iterations
+    // per sweep can certify with fewer refinement steps, each of which costs
+    // an FP64 residual pass)
+    for (int var : {0, 2, 3}) {
+    if (var >= 2 && k != 1) break;
     int prev_outer = 16;
-    for (int sweeps : (var == 2 ? block_sweeps : point_sweeps)) {
-        sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, 1, 256, 1, var, 0, sweeps, 1, 0, 0, var == 2 ? 4 : 0};
+    for (int sweeps : (var >= 2 ? block_sweeps : point_sweeps)) {
+        sptrsv_config_t c{SPTRSV_KERNEL_JACOBI, 1, 256, 1, std::min(var, 2), 0, sweeps, 1, 0, 0,
+                          var == 2 ? 4 : var == 3 ? 8 : 0};
         // the smallest number of refinement steps that certifies tol/8
         bool found = false;
         for (int outer = 1; outer <= std::min(16, prev_outer) && st == SPTRSV_OK; ++outer) {
@@ -1154,7 +1158,7 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
         for (int width : {1, 2}) {  // sweep tiles of 1024 or 2048 products
             c.lanes = width;
             // block sweeps iterate differently with another tile width
-            if (var == 2 && width == 2 && !certifies(c)) {
+            if (var >= 2 && width == 2 && !certifies(c)) {
                 if (st != SPTRSV_OK) break;
                 continue;
             }
@@ -1189,12 +1193,10 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
             bc = c;
         }
     }
-    // fewer sweeps in the steps after the first: they solve for a correction
-    // that needs only the remaining relative accuracy; stop at the first
-    // count that no longer certifies tol/8
     // block sweeps: other in-tile iteration counts on the best schedule
-    for (int inn : {2, 8}) {
+    for (int inn : {2, 8, 4}) {
         if (bc.variant != 2 || bt >= 1e30f || st != SPTRSV_OK) break;
+        if (inn == bc.inner) continue;
         sptrsv_config_t c = bc;
         c.inner = inn;
         if (!certifies(c)) {
@@ -1207,6 +1209,9 @@ int jacobi_tune(sp_mat_t* m, int k, int reps, sptrsv_config_t* best, float* best
             bc = c;
         }
     }
+    // fewer sweeps in the steps after the first: they solve for a correction
+    // that needs only the remaining relative accuracy; stop at the first
+    // count that no longer certifies tol/8
     for (int s2 = bc.sweeps - 1; s2 >= 2 && bc.outer > 1 && bt < 1e30f && st == SPTRSV_OK; --s2) {
         sptrsv_config_t c = bc;
         c.sweeps_next = s2;
