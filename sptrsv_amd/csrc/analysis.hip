@@ -589,10 +589,11 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
     // kernel is chosen for speed.  Rows of moderate length (mean >= 6, none
     // over 256 entries) take the lean one-row-per-wave pass with its lanes
     // covering a typical row and 4 resident waves per CU, which keeps the
-    // dependency hand-off short.  Short rows (mean < 6) take one lane per
-    // row on the general engine: 64 consecutive rows per wave keep many more
-    // rows in flight, which a natural-order walk of a grid-like matrix needs
-    // to find ready rows.  Other matrices keep the reference's tile rule.
+    // dependency hand-off short.  Short rows (mean < 6, none over 64
+    // entries) take one lane per row on the general engine: 64 consecutive
+    // rows per wave keep many more rows in flight, which a natural-order walk
+    // of a grid-like matrix needs to find ready rows.  Other matrices keep
+    // the reference's tile rule.
     int kind = KK_SIMPLE;
     unsigned dyn_lds = 0;
     const int avg = n > 0 ? m->nnz / n : 0;
@@ -602,7 +603,7 @@ static int analyse(dfr_analysis_info_t* info, sp_mat_t* m, MODE mode) {
         while (lanes < avg && lanes < kWave) lanes <<= 1;
         wg = 256;
         dyn_lds = lds_pad_for(4, wg);
-    } else if (avg < 6) {
+    } else if (avg < 6 && pv->max_row <= 64) {
         lanes = 1;
     }
     {
