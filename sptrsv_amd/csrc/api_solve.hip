@@ -1029,8 +1029,11 @@ int sptrsv_autotune(sp_mat_t* mat, dfr_analysis_info_t* info, int precision, int
     // general engine (several rows per wave)
     for (int L = lo; L <= hi; L <<= 1)
         for (int wg : wgs) shapes.push_back({SPTRSV_KERNEL_SIMPLE, L, wg, 1, 0, 0});
+    // in level order a wave's rows are independent, so a wave of 64 rows
+    // with one lane each is also tried (it wins on shallow matrices of many
+    // short rows, such as many independent systems in one matrix)
     if (has_order(info))
-        for (int L = lo; L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, 256, 1, 0, 0});
+        for (int L = (K == 1 ? 1 : lo); L <= hi; L <<= 1) shapes.push_back({SPTRSV_KERNEL_ORDER, L, 256, 1, 0, 0});
     if (has_waves(info))
         for (int wg : wgs) shapes.push_back({SPTRSV_KERNEL_MULTIROW, ref, wg, 1, 0, 0});
     // several right-hand sides: the persistent walk of the general engine
