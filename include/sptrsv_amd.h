@@ -316,7 +316,10 @@ typedef struct {
                          with the matrix), so the rows a wave holds are
                          adjacent in memory; b is read and x written in the
                          caller's order by the same kernels, and the results
-                         are bit-identical to reorder = 0 */
+                         are bit-identical to reorder = 0.  The copy takes
+                         16 bytes per entry, is made again for another
+                         analysis and dropped by sptrsv_update_values; not
+                         with variant 3 or the allocation-free solve */
 } sptrsv_config_t;
 
 /* Relative accuracy the JACOBI path certifies (default 1e-10). */
