@@ -8,8 +8,9 @@ MULTIROW, variants 0..2, FP64 and fused mixed, one and several right-hand
 sides, split long rows) and within 1e-12 (FP64) / 1e-10 (mixed) of the
 oracle; exact x == 1 on the 2D Laplacian factor; an upper-triangular handle;
 captured solves replayed bit-identical; the copy rebuilt after new values and
-for another analysis; and the refusals (SIMPLE, FP32, unfused mixed, the
-in-degree walk, the allocation-free solve).
+for another analysis; tiny matrices (one row, chains, diagonal-only rows);
+an autotuned copy serving automatic solves; and the refusals (SIMPLE, FP32,
+unfused mixed, the in-degree walk, the allocation-free solve).
 """
 from __future__ import annotations
 
@@ -17,6 +18,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
+from conftest import lower_from_rows
 from sptrsv_amd import _lib as L
 from sptrsv_amd import interface as I
 
@@ -215,4 +217,22 @@ def test_tuned_copy_serves_auto_solves():
         for f in fused:                          # an unfused call drops the copy, never fails on it
             x = solve(mat, info, prec, s.b, kernel=I.KERNEL_AUTO, fused=f)
             assert relerr(x, ref) <= tol, (prec, f)
+    I.free_device_analysis(mat, info)
+
+
+@pytest.mark.parametrize("rows", [[[]], [[], [0]], [[], [], [0, 1], [2], []],
+                                  [[], [0], [1], [2], [3], [0, 2, 4]]])
+def test_copy_on_tiny_matrices(rows):
+    print(f"level-ordered copy, {len(rows)} rows", flush=True)
+    ia, ja, a = lower_from_rows(rows)
+    n = len(rows)
+    b = np.linspace(1.0, 2.0, n)
+    mat = I.allocate_memory(ia, ja, a)
+    info = I.multirow_analysis_base_GPU(mat, I.ALL)
+    ref = O.fwd_f64(ia, ja, a, b)
+    for kern in (I.KERNEL_ORDER, I.KERNEL_MULTIROW):
+        for var in (0, 1, 2):
+            for prec, tol in ((I.PREC_F64, 1e-14), (I.PREC_MIXED, 1e-12)):
+                x = same_as_original(mat, info, prec, b, kernel=kern, lanes=4, wg=256, variant=var)
+                assert relerr(x, ref) <= tol, (kern, var, prec)
     I.free_device_analysis(mat, info)
