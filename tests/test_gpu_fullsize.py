@@ -81,13 +81,19 @@ def test_random_1e6():
                                                   cu_waves=4)),
         "mixed jacobi": (I.PREC_MIXED, dict(kernel=I.KERNEL_JACOBI, lanes=2, wg=256, variant=1, sweeps=12,
                                             outer=2)),
+        "f64 order persistent, level-ordered copy": (I.PREC_F64, dict(kernel=I.KERNEL_ORDER, lanes=16, wg=64,
+                                                                      variant=2, cu_waves=4, reorder=1)),
     }
+    got = {}
     for name, (prec, cfg) in cases.items():
         st, x = w.solve(prec, **cfg)
         err = relerr(x, w.ref)
         say(f"{name}: status {st}, rel err {err:.2e}")
         assert st == 0, (name, L.last_error())
         assert err <= (1e-12 if prec == I.PREC_F64 else 1e-10), (name, err)
+        got[name] = x
+    a, c = got["f64 order persistent"], got["f64 order persistent, level-ordered copy"]
+    assert np.array_equal(a.view(np.uint64), c.view(np.uint64))   # the copy changes no bit
     w.close()
 
 
