@@ -691,6 +691,10 @@ __global__ __launch_bounds__(WG) void k_order(SolveArgs A) {
 __device__ __forceinline__ int multirow_item(const SolveArgs& A, int t) { return A.items ? A.items[t] : t; }
 __device__ __forceinline__ int multirow_items(const SolveArgs& A) { return A.items ? A.n_items : A.n_waves; }
 
+// A segment item of a split long row (defined with the lean walk below).
+template <class P, int U>
+__device__ __forceinline__ void fast_segment(const SolveArgs& A, int lane, int g);
+
 // (level, class) packed waves (multirow): wave w solves the rows at iorder
 // positions [wbase[w], wbase[w+1]) with wsize[w] lanes per row.  Rows of one
 // wave share a level, so they never depend on each other.  Segment items
