@@ -358,9 +358,14 @@ int csr_L_solve_mixed(sp_mat_t* mat, dfr_analysis_info_t* info, const double* b,
  * hipFree or other allocating routine.  n * k above the maximum is
  * SPTRSV_E_CAPACITY; a MULTIROW solve that splits long rows (one right-hand
  * side) also keeps its segment partials there, after the published words,
- * and must fit both.  Sync-free kernels only: a configuration that resolves
- * to JACOBI is SPTRSV_E_INVALID_ARG, as is a handle stored in reversed order
- * (allocate_memory_upper, or the transpose of a lower-triangular handle).
+ * and must fit both.  Sync-free kernels only: a named JACOBI configuration
+ * is SPTRSV_E_INVALID_ARG, as is a handle stored in reversed order
+ * (allocate_memory_upper, or the transpose of a lower-triangular handle), the
+ * in-degree walk (variant 3) and a named level-ordered copy (reorder = 1).
+ * An automatic call (cfg NULL or kernel AUTO) takes the tuned shape, with
+ * two substitutions: a tuned JACOBI choice becomes the default sync-free
+ * shape, and a tuned level-ordered copy walks the matrix as given (same
+ * bits).
  * Each call re-initialises the scratch it uses.  Every call on one GPU
  * shares that GPU's scratch, so calls that overlap in time on one GPU (two
  * streams, two threads) race and must be serialised by the caller. */

@@ -881,16 +881,25 @@ int sptrsv_solve_mixed_noalloc(sp_mat_t* mat, dfr_analysis_info_t* info, const s
     if (n == 0) return SPTRSV_OK;
     sptrsv_config_t c;
     SPT_TRY(resolve_config(mat, info, SPTRSV_PREC_MIXED, k, cfg, &c));
-    if (c.kernel == SPTRSV_KERNEL_JACOBI)
-        return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve runs the sync-free kernels only "
-                                          "(JACOBI keeps per-matrix workspaces); name SIMPLE, ORDER or MULTIROW");
+    const bool named = cfg && cfg->kernel != SPTRSV_KERNEL_AUTO;
+    if (c.kernel == SPTRSV_KERNEL_JACOBI) {
+        if (named)
+            return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve runs the sync-free kernels only "
+                                              "(JACOBI keeps per-matrix workspaces); name SIMPLE, ORDER or MULTIROW");
+        // a tuned JACOBI choice: the default sync-free shape instead
+        SPT_TRY(default_config(mat, info, SPTRSV_PREC_MIXED, k, &c));
+    }
     c.fused = 1;
     if (c.variant == 3)
         return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve does not run the in-degree walk (its "
                                           "counters are a per-matrix workspace)");
-    if (c.reorder == 1)
-        return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve does not walk a level-ordered copy (the "
-                                          "copy is a per-matrix workspace)");
+    if (c.reorder == 1) {
+        // a tuned choice walks the matrix as given instead; a named one is refused
+        if (named && cfg->reorder == 1)
+            return fail(SPTRSV_E_INVALID_ARG, "the allocation-free solve does not walk a level-ordered copy (the "
+                                              "copy is a per-matrix workspace)");
+        c.reorder = 0;
+    }
     // the published words, then (split long rows) the segment partials from
     // the next 256-byte boundary
     const size_t pub_words = (size_t)n * k;

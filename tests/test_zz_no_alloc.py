@@ -89,6 +89,39 @@ def test_noalloc_several_rhs():
     I.free_device_analysis(mat, info)
 
 
+def test_noalloc_after_tuning():
+    """After autotuning, an automatic call takes the tuned shape; a tuned
+    level-ordered copy (a per-matrix workspace) is walked as the matrix as
+    given, with the same bits, and a tuned JACOBI choice becomes the default
+    sync-free shape."""
+    print("no-alloc after autotune: 16 random batch members as one matrix", flush=True)
+    mem = [I.generate_batch_member(m, 2000, 200000, seed=5) for m in range(1, 32, 2)]
+    n = sum(s.n for s in mem)
+    ia = np.zeros(n + 1, np.int64)
+    ja, a, b = [], [], []
+    r0 = e0 = 0
+    for s in mem:
+        ia[r0 + 1:r0 + s.n + 1] = s.ia[1:].astype(np.int64) - s.ia[0] + e0
+        ja.append(s.ja + r0)
+        a.append(s.a)
+        b.append(s.b)
+        r0 += s.n
+        e0 += s.nnz
+    ia, ja, a, b = ia.astype(np.int32), np.concatenate(ja), np.concatenate(a), np.concatenate(b)
+    assert n <= L.NOALLOC_MAX_WORDS
+    mat = I.allocate_memory(ia, ja, a)
+    info = I.multirow_analysis_base_GPU(mat, I.ALL)
+    cfg, _ = I.autotune(mat, info, I.PREC_MIXED, reps=2)
+    print(f"  tuned kernel {cfg.kernel} variant {cfg.variant} reorder {cfg.reorder}", flush=True)
+    ref = O.fwd_f64(ia, ja, a, b)
+    for _ in range(2):
+        xa, xb = both(mat, info, b, 1, None)
+        if cfg.kernel != I.KERNEL_JACOBI:    # same sync-free shape: same bits
+            assert np.array_equal(xa.view(np.uint64), xb.view(np.uint64))
+        assert relerr(xb, ref) <= 1e-10      # (a tuned JACOBI becomes the default sync-free shape)
+    I.free_device_analysis(mat, info)
+
+
 def test_noalloc_rejects_what_it_cannot_do():
     print("no-alloc argument checks", flush=True)
     n = L.NOALLOC_MAX_WORDS // 64 + 1
